@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X ray-tracing hot path.
+
+One step = one render of the RTOW final scene with every sphere tessellated
+at level 6 (7.95M world-space triangles, seed 0), 1920x1080, 500 spp, depth 16
+(the render of optx/camera_r.cu + optx/optics_r.cu under one launch), with the
+acceleration structure already built and resident in HBM.
+
+Prints ONE JSON line on stdout (progress goes to stderr).  value = traced rays
+per second (hits + misses, counted as the reference's rays-per-pixel AOV) in
+Mrays/s; samples/s beside it.  roofline: algorithmic bytes per ray (BVH node
+visits x 64 B + triangle tests x 48 B + shading fetch 80 B, counted by a
+statistics launch) x rays/s against the 8 TB/s HBM peak.  cpu_baseline: the
+oracle's renderer (C, same scene, same semantics) on the host cores over a
+bounded sample of rows.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c3|c2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s + samples/s at 1080p 500spp (RMSE-matched); batched speedup; bwd/fwd ratio"
+WORKLOADS = {
+    # name: (level, w, h, spp, depth)
+    "c3": (6, 1920, 1080, 500, 16),
+    "c2": (4, 1280, 720, 50, 16),
+}
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, SHADE_BYTES = 64, 48, 80
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(level, w, h, spp, depth, seconds, threads):
+    """The oracle renderer on host cores over a bounded sample of rows."""
+    import oracle
+    from rtxplay_amd.scene import sphere_things
+    things = sphere_things(0, 11)
+    meshes = {level: oracle.sphere_mesh(1.0, level)}
+    tris, inst = oracle.flatten(meshes, things, [level] * len(things))
+    log("cpu baseline: oracle BVH over %d triangles" % len(tris))
+    sc = oracle.Scene(tris, inst, things)
+    cam = oracle.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0)
+    order = np.random.default_rng(0).permutation(h // threads)
+    rays, rows, t0 = 0, 0, time.perf_counter()
+    for band in order:
+        y0 = int(band) * threads
+        rays += sc.render(cam, w, h, spp, depth, y0=y0, y1=y0 + threads, nthreads=threads)["rays"]
+        rows += threads
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "%d random rows of %dx%d at %d spp depth %d on the same level-%d scene (%.1f s, %d rays)"
+                      % (rows, w, h, spp, depth, level, dt, rays)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stats-spp", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (pixels would be sharded "
+              "by row bands with the BVH replicated per GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import rtxplay_amd as rtx
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene
+
+    level, w, h, spp, depth = WORKLOADS[args.workload]
+    if rtx.device_count() < 1:
+        sys.exit("bench.py needs a GPU")
+    t0 = time.perf_counter()
+    scene = RTOWScene(seed=0, half=11, level=level)
+    scene.load()
+    handle = scene.build()
+    st = scene.stats()
+    log("scene: %d things, %d triangles, %d nodes, depth %d, SAH %.1f, build %.1f ms (host %.1f s)"
+        % (len(scene.things_spec), st["num_prims"], st["num_nodes"], st["max_depth"], st["sah_cost"],
+           st["build_ms"], time.perf_counter() - t0))
+
+    lp = rtx.LpGeneral()
+    lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, depth, handle
+    lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0).c
+    launcher = rtx.Launcher(scene, lp)
+    stream = Stream()
+    for i in range(args.warmup):
+        launcher.ignite(stream=stream, sync=False)
+        stream.synchronize()
+        log("warmup %d done" % i)
+    rtx.synchronize()
+    e0, e1 = Event(), Event()
+    wall0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(args.steps):
+        launcher.ignite(stream=stream, sync=False)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = e0.elapsed_ms(e1) / args.steps
+    rpp = launcher.rays_per_pixel()
+    rays = int(rpp.sum(dtype=np.uint64))
+    log("timed %d steps: %.1f ms/step (wall %.2f s), %d rays/frame" % (args.steps, ms, wall, rays))
+    mrays = rays / (ms * 1e-3) / 1e6
+    samples = w * h * spp / (ms * 1e-3)
+
+    # bytes per ray from a statistics launch at reduced spp (per-ray averages)
+    lp.spp = args.stats_spp
+    nodes, tris = launcher.ignite_counted(stream=stream)
+    srays = int(launcher.rays_per_pixel().sum(dtype=np.uint64))
+    lp.spp = spp
+    n_node = float(nodes.sum(dtype=np.uint64)) / srays
+    n_tri = float(tris.sum(dtype=np.uint64)) / srays
+    io = (12 + 4 + 12 + 12) * w * h / rays
+    b_ray = n_node * NODE_BYTES + n_tri * TRI_BYTES + SHADE_BYTES + io
+    achieved = b_ray * rays / (ms * 1e-3) / 1e9
+    log("per ray: %.1f node visits, %.1f triangle tests -> %.0f B/ray, %.0f GB/s" % (n_node, n_tri, b_ray, achieved))
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "render_%s_traffic.json" % args.workload)
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(level, w, h, spp, depth, args.cpu_seconds, threads)
+        log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
+
+    out = {
+        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "rtow-final-scene level-%d spheres (%d triangles), %dx%d, %d spp, depth %d, seed 0"
+                               % (level, st["num_prims"], w, h, spp, depth),
+                   "image": [w, h], "spp": spp, "depth": depth, "triangles": st["num_prims"],
+                   "parallelism": "single-gpu"},
+        "samples_per_s": round(samples, 1), "Msamples_per_s": round(samples / 1e6, 2),
+        "rays_per_frame": rays, "rays_per_sample": round(rays / (w * h * spp), 4),
+        "bvh": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "max_depth": st["max_depth"],
+                "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 2),
+                     "tri_tests_per_ray": round(n_tri, 2)},
+        "cpu_baseline": cpu,
+        "batched_speedup": None, "bwd_fwd_ratio": None,
+    }
+    print(json.dumps(out), flush=True)
+    launcher.free()
+    scene.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+/*
+ * rtx.h -- C-ABI of the MI355X ray-tracing hot path (librtx.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Device pointers are
+ * HIP device addresses; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every call returns an rtx_status; on failure rtx_last_error()
+ * returns a message for the calling thread.
+ *
+ * Each entry point names the reference interface it replaces.  Citations are
+ * relative to the reference repository (otabuzzman/RTXplay); optx/... is the
+ * OptiX program.
+ */
+#ifndef RTX_H
+#define RTX_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------- */
+typedef enum {
+    RTX_SUCCESS = 0,
+    RTX_ERROR_INVALID_VALUE = 1,   /* bad argument (null pointer, size 0, spp 0, ...) */
+    RTX_ERROR_INVALID_HANDLE = 2,  /* unknown traversable handle / id out of range */
+    RTX_ERROR_NOT_BUILT = 3,       /* scene has no acceleration structure yet */
+    RTX_ERROR_OUT_OF_MEMORY = 4,
+    RTX_ERROR_HIP = 5,             /* a HIP runtime call failed; message has the text */
+    RTX_ERROR_LIMIT = 6            /* a documented size limit is exceeded */
+} rtx_status;
+
+/* message of the last failed call on this thread ("" if none) */
+const char* rtx_last_error(void);
+
+/* ---- value types, layout-compatible with the reference's CUDA structs ----- */
+typedef struct { float x, y, z; } rtx_float3;           /* CUDA float3 */
+typedef struct { uint32_t x, y, z; } rtx_uint3;         /* CUDA uint3  */
+typedef struct { uint8_t x, y, z, w; } rtx_uchar4;      /* CUDA uchar4 */
+
+/* Optics / Thing: optx/thing.h:17-52 (same field order, types and union) */
+enum { RTX_OPTICS_DIFFUSE = 0, RTX_OPTICS_REFLECT = 1, RTX_OPTICS_REFRACT = 2, RTX_OPTICS_NUM = 3 };
+typedef struct { rtx_float3 albedo; } rtx_diffuse;
+typedef struct { rtx_float3 albedo; float fuzz; } rtx_reflect;
+typedef struct { float index; } rtx_refract;
+typedef struct {
+    int32_t type;
+    union { rtx_diffuse diffuse; rtx_reflect reflect; rtx_refract refract; };
+} rtx_optics;
+typedef struct {
+    rtx_float3* vces;   /* non-owning device pointer, filled by rtx_thing_add */
+    rtx_uint3*  ices;   /* non-owning device pointer, filled by rtx_thing_add */
+    rtx_optics  optics;
+} rtx_thing;
+
+/* Camera: optx/camera.h:81-93 (private members of the class, as a POD) */
+typedef struct {
+    rtx_float3 eye, pat, vup;
+    float fov, aspratio, aperture, fostance;
+    rtx_float3 u, v, w;
+    rtx_float3 hvec, wvec, dvec;
+} rtx_camera;
+
+/* Launch parameters: optx/rtwo.h:26-52, same field order and types with
+ * OptixTraversableHandle -> uint64_t (a handle from rtx_scene_build). */
+typedef struct {
+    rtx_uchar4* image;          /* not written by rtx_launch (postproc output) */
+    uint32_t    image_w;
+    uint32_t    image_h;
+    rtx_float3* rawRGB;         /* w*h mean colour, clamped to [0,1] */
+    rtx_float3* normals;        /* optional (NULL: not written) */
+    rtx_float3* albedos;        /* optional (NULL: not written) */
+    rtx_camera  camera;
+    uint32_t    spp;
+    uint32_t    depth;          /* max rays per sample (>=1) */
+    uint64_t    is_handle;
+    uint32_t*   rpp;            /* rays per pixel; optional (NULL: not written) */
+    bool        picker;
+    uint32_t    pick_x;
+    uint32_t    pick_y;
+    uint32_t*   pick_id;
+} rtx_lp_general;
+
+/* A ray for the closest-hit query; 32 bytes. */
+typedef struct { rtx_float3 ori; float tmin; rtx_float3 dir; float tmax; } rtx_ray;
+/* Closest hit: t, barycentrics (u,v) weighting vertices 1 and 2 (as
+ * optixGetTriangleBarycentrics), instance id and primitive index within the
+ * instance's mesh (as optixGetInstanceId / optixGetPrimitiveIndex).  On a
+ * miss instance_id == primitive_index == 0xFFFFFFFF and t = +inf. */
+typedef struct { float t, u, v; uint32_t instance_id; uint32_t primitive_index; uint32_t prim; uint32_t pad[2]; } rtx_hit;
+
+typedef struct rtx_ctx rtx_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+/* Replaces optixInit + optixDeviceContextCreate (optx/rtwo.cxx:115-128) and
+ * owns what Scene owns in the reference (optx/scene.h:37-62). */
+rtx_status rtx_ctx_create(int device, rtx_ctx** ctx);
+rtx_status rtx_ctx_destroy(rtx_ctx* ctx);
+
+/* ---- camera ------------------------------------------------------------- */
+/* Camera::set, optx/camera.h:30-48 (host side, FP32) */
+rtx_status rtx_camera_set(rtx_camera* cam, rtx_float3 eye, rtx_float3 pat, rtx_float3 vup,
+                          float fov, float aspratio, float aperture, float fostance);
+
+/* ---- geometry ------------------------------------------------------------ */
+/* Sphere(radius, ndiv).mesh(), optx/sphere.cxx:17-106 (host).  With vces/ices
+ * NULL only the counts are returned.  Capacities in elements. */
+rtx_status rtx_sphere_mesh(float radius, uint32_t ndiv, rtx_float3* vces, uint32_t vcap,
+                           uint32_t* nv, rtx_uint3* ices, uint32_t icap, uint32_t* ni);
+
+/* Scene::add(Object&), optx/scene.cxx:38-167: uploads one mesh (the caller
+ * concatenates an Object's submeshes) and returns its id. */
+rtx_status rtx_mesh_add(rtx_ctx* ctx, const rtx_float3* vces, uint32_t nv,
+                        const rtx_uint3* ices, uint32_t ni, uint32_t* mesh_id);
+
+/* Scene::add(Thing&, object), optx/scene.cxx:169-196: new instance of a mesh
+ * with identity transform; fills thing->vces/ices with the mesh's device
+ * buffers.  Returns the instance (thing) id. */
+rtx_status rtx_thing_add(rtx_ctx* ctx, rtx_thing* thing, uint32_t mesh_id, uint32_t* thing_id);
+
+/* Scene::set / Scene::get, optx/scene.cxx:198-225: 3x4 row-major object-to-
+ * world transform.  An id out of range returns RTX_ERROR_INVALID_HANDLE (the
+ * reference returns false). */
+rtx_status rtx_thing_set(rtx_ctx* ctx, uint32_t thing_id, const float transform[12]);
+rtx_status rtx_thing_get(rtx_ctx* ctx, uint32_t thing_id, float transform[12]);
+/* Scene::operator[], optx/scene.h:21 */
+rtx_status rtx_thing_at(rtx_ctx* ctx, uint32_t thing_id, rtx_thing* thing);
+rtx_status rtx_thing_count(rtx_ctx* ctx, uint32_t* n);
+
+/* Scene::build, optx/scene.cxx:227-273: flattens every instance to world
+ * space on the device and builds the acceleration structure there (binned
+ * SAH BVH).  Synchronous.  Returns the traversable handle. */
+rtx_status rtx_scene_build(rtx_ctx* ctx, void* stream, uint64_t* is_handle);
+/* Scene::update, optx/scene.cxx:275-294: brings the acceleration structure up
+ * to date with rtx_thing_set changes (rebuild). Synchronous. */
+rtx_status rtx_scene_update(rtx_ctx* ctx, uint64_t is_handle, void* stream);
+
+/* Statistics of a built scene (for reports and tests). */
+typedef struct {
+    uint32_t num_prims, num_nodes, num_leaves, max_depth, max_leaf;
+    float    sah_cost;          /* SAH cost, C_trav = 1, C_isect = 1 */
+    float    build_ms;          /* device time of the last build */
+    uint64_t device_bytes;      /* nodes + leaf-ordered triangles + instance data */
+} rtx_scene_stats;
+rtx_status rtx_scene_stats_get(rtx_ctx* ctx, uint64_t is_handle, rtx_scene_stats* st);
+
+/* Copies the world-space triangles (9 floats each, original primitive order:
+ * instance-major, mesh triangle order) and per-triangle instance ids to host
+ * memory.  Either pointer may be NULL. */
+rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, uint32_t* tri_inst);
+
+/* Miss record (SBT miss data), optx/rtwo.cxx:110-111 and :474-489.  Default {.5,.7,1}. */
+rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
+
+/* ---- launch -------------------------------------------------------------- */
+/* Launcher::ignite, optx/launcher.cxx:49-77 (optixLaunch of the raygen program
+ * optx/camera_r.cu:23-121 with the closest-hit programs optx/optics_r.cu).
+ * Asynchronous on `stream` (the C++ Launcher syncs, as the reference does).
+ * once != 0 launches the 1x1 picker path (lp->picker). */
+rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
+
+/* Statistics launch: the same render as rtx_launch, which also writes per
+ * pixel the number of BVH node visits and triangle tests of all its rays
+ * (device uint64 arrays of image_w*image_h).  No counterpart in the
+ * reference; it feeds the bytes-per-ray model of the benchmark. */
+rtx_status rtx_launch_counted(rtx_ctx* ctx, const rtx_lp_general* lp, uint64_t* node_visits,
+                              uint64_t* tri_tests, void* stream);
+
+/* Closest-hit query for a batch of rays (the optixTrace call the programs
+ * make, optx/camera_r.cu:82-99), for tests and hybrid callers.  rays and hits
+ * are device pointers.  Asynchronous. */
+rtx_status rtx_trace(rtx_ctx* ctx, uint64_t is_handle, const rtx_ray* rays, uint32_t n,
+                     rtx_hit* hits, void* stream);
+
+/* Device Frand48 (optx/frand48.h:17-35): k draws after init(seed) for each of
+ * n seeds into out[n*k] (device pointers).  Asynchronous. */
+rtx_status rtx_rng_draws(const uint32_t* seeds, uint32_t n, uint32_t k, float* out, void* stream);
+
+/* ---- post-process -------------------------------------------------------- */
+/* pp_sRGB / pp_none, optx/postproc.cu:49-61 (the reference launches on the null
+ * stream and reports nothing; these take a stream and return a status). */
+rtx_status rtx_pp_srgb(const rtx_float3* src, rtx_uchar4* dst, int w, int h, void* stream);
+rtx_status rtx_pp_none(const rtx_float3* src, rtx_uchar4* dst, int w, int h, void* stream);
+
+/* ---- device memory plumbing for callers without a HIP binding ------------ */
+rtx_status rtx_malloc(void** dptr, size_t bytes);
+rtx_status rtx_free(void* dptr);
+rtx_status rtx_memcpy_htod(void* dst, const void* src, size_t bytes);
+rtx_status rtx_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+rtx_status rtx_memset(void* dptr, int value, size_t bytes);
+rtx_status rtx_synchronize(void);
+rtx_status rtx_device_count(int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTX_H */

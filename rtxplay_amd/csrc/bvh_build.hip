@@ -1,0 +1,602 @@
+// bvh_build.hip -- on-device acceleration structure build.
+//
+// Replaces the OptiX GAS/IAS builds of optx/scene.cxx:38-273 (optixAccelBuild)
+// with: instance flattening to world space, a top-down binned SAH BVH built
+// level by level on the device (Wald 2007, "On fast construction of SAH-based
+// bounding volume hierarchies"), triangle reordering into leaf order, and a
+// conversion to the 64-byte two-child node layout the traversal reads.
+//
+// Determinism: every level is processed in task order, child nodes and tasks
+// are placed by an exclusive scan (no atomics decide positions) and the
+// partition is stable, so the same input always gives the same BVH.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+
+namespace rtx {
+
+namespace {
+
+constexpr int kBins = 32;
+constexpr int kBuildBlock = 256;
+constexpr int kScanBlock = 1024;
+// SAH constants (cost of one node visit vs one triangle test)
+constexpr float kCostTrav = 1.0f;
+constexpr float kCostIsect = 1.0f;
+
+struct BuildTask {
+    uint32_t node, begin, end, depth;
+    float4 clo, chi;   // centroid bounds of the task's primitives
+};
+
+struct SplitOut {
+    uint32_t kind;     // 0 leaf, 1 split
+    uint32_t nleft;
+    uint32_t pad0, pad1;
+    float4 clo_l, chi_l, clo_r, chi_r;
+};
+
+// binary node during the build: lo.w/hi.w carry the links as uint bits
+//   inner: lo.w = left child (right = left + 1), hi.w = 0
+//   leaf : lo.w = first leaf-ordered primitive, hi.w = count (>= 1)
+struct BNode {
+    float4 lo, hi;
+};
+
+struct BuildStats {
+    uint32_t leaves, max_depth, max_leaf, pad;
+    double sah;
+};
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__device__ __forceinline__ float area3(float lx, float ly, float lz, float hx, float hy, float hz) {
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    if (!(dx >= 0.f) || !(dy >= 0.f) || !(dz >= 0.f)) return 0.f;
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+
+__device__ __forceinline__ float comp4(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ int bin_of(float c, float lo, float scale) {
+    int b = (int)((c - lo) * scale);
+    return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
+}
+
+// ---------------------------------------------------------------------------
+// flatten: one thread per world-space triangle (instance-major order)
+// world = R v + t evaluated as ((v.x*t0 + v.y*t1) + v.z*t2) + t3, one
+// rounding per operation (optx/optics_r.cu:61-75 with contraction ruled out)
+// ---------------------------------------------------------------------------
+__global__ void k_flatten(const InstBuild* __restrict__ insts, uint32_t ninst,
+                          const MeshRef* __restrict__ meshes, uint32_t N,
+                          float4* __restrict__ tri_world, float4* __restrict__ plo,
+                          float4* __restrict__ phi) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    uint32_t lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (insts[mid].tri_base <= p) lo = mid; else hi = mid - 1;
+    }
+    const InstBuild& in = insts[lo];
+    const uint32_t j = p - in.tri_base;
+    const MeshRef m = meshes[in.mesh];
+    float3 w[3];
+    for (int c = 0; c < 3; c++) {
+        const uint32_t vi = m.ices[3 * j + c];
+        const float vx = m.vces[3 * vi + 0], vy = m.vces[3 * vi + 1], vz = m.vces[3 * vi + 2];
+        float r[3];
+        for (int k = 0; k < 3; k++) {
+            float acc = __fmul_rn(vx, in.xf[4 * k + 0]);
+            acc = __fadd_rn(acc, __fmul_rn(vy, in.xf[4 * k + 1]));
+            acc = __fadd_rn(acc, __fmul_rn(vz, in.xf[4 * k + 2]));
+            r[k] = __fadd_rn(acc, in.xf[4 * k + 3]);
+        }
+        w[c] = make_float3(r[0], r[1], r[2]);
+    }
+    tri_world[3 * (size_t)p + 0] = make_float4(w[0].x, w[0].y, w[0].z, __uint_as_float(p));
+    tri_world[3 * (size_t)p + 1] = make_float4(w[1].x, w[1].y, w[1].z, __uint_as_float(lo));
+    tri_world[3 * (size_t)p + 2] = make_float4(w[2].x, w[2].y, w[2].z, __uint_as_float(j));
+    plo[p] = make_float4(fminf(fminf(w[0].x, w[1].x), w[2].x), fminf(fminf(w[0].y, w[1].y), w[2].y),
+                         fminf(fminf(w[0].z, w[1].z), w[2].z), 0.f);
+    phi[p] = make_float4(fmaxf(fmaxf(w[0].x, w[1].x), w[2].x), fmaxf(fmaxf(w[0].y, w[1].y), w[2].y),
+                         fmaxf(fmaxf(w[0].z, w[1].z), w[2].z), 0.f);
+}
+
+// root task: centroid bounds of all primitives (one block, strided)
+__global__ void k_root_task(const float4* __restrict__ plo, const float4* __restrict__ phi, uint32_t N,
+                            BuildTask* __restrict__ task, uint32_t* __restrict__ idx) {
+    __shared__ uint32_t s_lo[3], s_hi[3];
+    if (threadIdx.x < 3) { s_lo[threadIdx.x] = 0xFFFFFFFFu; s_hi[threadIdx.x] = 0u; }
+    __syncthreads();
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
+        const float4 a = plo[p], b = phi[p];
+        const float c[3] = {.5f * (a.x + b.x), .5f * (a.y + b.y), .5f * (a.z + b.z)};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = min(lo[k], f2o(c[k]));
+            hi[k] = max(hi[k], f2o(c[k]));
+        }
+    }
+    for (int k = 0; k < 3; k++) { atomicMin(&s_lo[k], lo[k]); atomicMax(&s_hi[k], hi[k]); }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BuildTask t;
+        t.node = 0; t.begin = 0; t.end = N; t.depth = 0;
+        t.clo = make_float4(o2f(s_lo[0]), o2f(s_lo[1]), o2f(s_lo[2]), 0.f);
+        t.chi = make_float4(o2f(s_hi[0]), o2f(s_hi[1]), o2f(s_hi[2]), 0.f);
+        *task = t;
+    }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ idx, uint32_t N) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < N) idx[p] = p;
+}
+
+// block-wide exclusive scan of one value per thread; returns the prefix and
+// the block total.  `ws` holds one slot per wave plus one.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63u) ws[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < BLOCK / 64; i++) { const uint32_t t = ws[i]; ws[i] = run; run += t; }
+        ws[BLOCK / 64] = run;
+    }
+    __syncthreads();
+    const uint32_t excl = ws[wid] + x - v;
+    total = ws[BLOCK / 64];
+    __syncthreads();
+    return excl;
+}
+
+// ---------------------------------------------------------------------------
+// one BVH level: one block per task.  Bins the task's primitives (3 axes x 32
+// bins; per-thread runs of equal bins are merged before the LDS atomics),
+// evaluates SAH, chooses leaf / SAH split / median split, partitions stably.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBuildBlock) void k_level(
+        const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+        const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+        BNode* __restrict__ bnodes, SplitOut* __restrict__ out) {
+    __shared__ uint32_t s_cnt[3][kBins];
+    __shared__ uint32_t s_b[3][kBins][12];   // lo.xyz, hi.xyz, clo.xyz, chi.xyz (ordered ints)
+    __shared__ float s_rarea[3][kBins];
+    __shared__ uint32_t s_rcnt[3][kBins];
+    __shared__ float s_best_cost[3];
+    __shared__ int s_best_bin[3];
+    __shared__ int s_axis, s_bin, s_kind;
+    __shared__ uint32_t s_nleft;
+    __shared__ uint32_t s_ws[kBuildBlock / 64 + 1];
+
+    const BuildTask task = tasks[blockIdx.x];
+    const uint32_t n = task.end - task.begin;
+    const float cl[3] = {task.clo.x, task.clo.y, task.clo.z};
+    const float ch[3] = {task.chi.x, task.chi.y, task.chi.z};
+    float scale[3];
+    for (int k = 0; k < 3; k++) {
+        const float ext = ch[k] - cl[k];
+        scale[k] = ext > 0.f ? (float)kBins * (1.f - 1e-6f) / ext : 0.f;
+    }
+
+    for (int i = threadIdx.x; i < 3 * kBins; i += kBuildBlock) {
+        const int ax = i / kBins, b = i % kBins;
+        s_cnt[ax][b] = 0;
+        for (int k = 0; k < 12; k++) s_b[ax][b][k] = (k < 3 || (k >= 6 && k < 9)) ? 0xFFFFFFFFu : 0u;
+    }
+    __syncthreads();
+
+    // ---- binning: each thread walks a contiguous run of the task's range
+    {
+        const uint32_t chunk = (n + kBuildBlock - 1) / kBuildBlock;
+        const uint32_t b0 = task.begin + min(n, threadIdx.x * chunk);
+        const uint32_t b1 = task.begin + min(n, (threadIdx.x + 1) * chunk);
+        int cur[3] = {-1, -1, -1};
+        uint32_t cnt[3] = {0, 0, 0};
+        uint32_t acc[3][12];
+        auto flush = [&](int ax) {
+            if (cur[ax] < 0 || cnt[ax] == 0) return;
+            const int b = cur[ax];
+            atomicAdd(&s_cnt[ax][b], cnt[ax]);
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&s_b[ax][b][k], acc[ax][k]);
+                atomicMax(&s_b[ax][b][3 + k], acc[ax][3 + k]);
+                atomicMin(&s_b[ax][b][6 + k], acc[ax][6 + k]);
+                atomicMax(&s_b[ax][b][9 + k], acc[ax][9 + k]);
+            }
+        };
+        for (uint32_t i = b0; i < b1; i++) {
+            const uint32_t p = idx[i];
+            const float4 a = plo[p], h = phi[p];
+            const float c[3] = {.5f * (a.x + h.x), .5f * (a.y + h.y), .5f * (a.z + h.z)};
+            const uint32_t v[12] = {f2o(a.x), f2o(a.y), f2o(a.z), f2o(h.x), f2o(h.y), f2o(h.z),
+                                    f2o(c[0]), f2o(c[1]), f2o(c[2]), f2o(c[0]), f2o(c[1]), f2o(c[2])};
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                const int b = bin_of(c[ax], cl[ax], scale[ax]);
+                if (b != cur[ax]) {
+                    flush(ax);
+                    cur[ax] = b;
+                    cnt[ax] = 0;
+#pragma unroll
+                    for (int k = 0; k < 12; k++) acc[ax][k] = v[k];
+                }
+                cnt[ax]++;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    acc[ax][k] = min(acc[ax][k], v[k]);
+                    acc[ax][3 + k] = max(acc[ax][3 + k], v[3 + k]);
+                    acc[ax][6 + k] = min(acc[ax][6 + k], v[6 + k]);
+                    acc[ax][9 + k] = max(acc[ax][9 + k], v[9 + k]);
+                }
+            }
+        }
+        flush(0); flush(1); flush(2);
+    }
+    __syncthreads();
+
+    // ---- SAH sweep, one thread per axis
+    if (threadIdx.x < 3) {
+        const int ax = threadIdx.x;
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+        uint32_t c = 0;
+        for (int b = kBins - 1; b > 0; b--) {
+            if (s_cnt[ax][b]) {
+                lx = fminf(lx, o2f(s_b[ax][b][0])); ly = fminf(ly, o2f(s_b[ax][b][1])); lz = fminf(lz, o2f(s_b[ax][b][2]));
+                hx = fmaxf(hx, o2f(s_b[ax][b][3])); hy = fmaxf(hy, o2f(s_b[ax][b][4])); hz = fmaxf(hz, o2f(s_b[ax][b][5]));
+                c += s_cnt[ax][b];
+            }
+            s_rarea[ax][b] = area3(lx, ly, lz, hx, hy, hz);
+            s_rcnt[ax][b] = c;
+        }
+        lx = FLT_MAX; ly = FLT_MAX; lz = FLT_MAX; hx = -FLT_MAX; hy = -FLT_MAX; hz = -FLT_MAX;
+        c = 0;
+        float best = FLT_MAX;
+        int bestb = -1;
+        for (int b = 0; b < kBins - 1; b++) {
+            if (s_cnt[ax][b]) {
+                lx = fminf(lx, o2f(s_b[ax][b][0])); ly = fminf(ly, o2f(s_b[ax][b][1])); lz = fminf(lz, o2f(s_b[ax][b][2]));
+                hx = fmaxf(hx, o2f(s_b[ax][b][3])); hy = fmaxf(hy, o2f(s_b[ax][b][4])); hz = fmaxf(hz, o2f(s_b[ax][b][5]));
+                c += s_cnt[ax][b];
+            }
+            const uint32_t rc = s_rcnt[ax][b + 1];
+            if (c == 0 || rc == 0) continue;
+            const float cost = area3(lx, ly, lz, hx, hy, hz) * (float)c + s_rarea[ax][b + 1] * (float)rc;
+            if (cost < best) { best = cost; bestb = b; }
+        }
+        s_best_cost[ax] = best;
+        s_best_bin[ax] = bestb;
+    }
+    __syncthreads();
+
+    if (threadIdx.x == 0) {
+        // node bounds: union of the axis-0 bins
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+        for (int b = 0; b < kBins; b++) {
+            if (!s_cnt[0][b]) continue;
+            lx = fminf(lx, o2f(s_b[0][b][0])); ly = fminf(ly, o2f(s_b[0][b][1])); lz = fminf(lz, o2f(s_b[0][b][2]));
+            hx = fmaxf(hx, o2f(s_b[0][b][3])); hy = fmaxf(hy, o2f(s_b[0][b][4])); hz = fmaxf(hz, o2f(s_b[0][b][5]));
+        }
+        BNode nd;
+        nd.lo = make_float4(lx, ly, lz, 0.f);
+        nd.hi = make_float4(hx, hy, hz, 0.f);
+        bnodes[task.node] = nd;
+        int axis = -1;
+        float best = FLT_MAX;
+        for (int ax = 0; ax < 3; ax++)
+            if (s_best_bin[ax] >= 0 && s_best_cost[ax] < best) { best = s_best_cost[ax]; axis = ax; }
+        const float A = area3(lx, ly, lz, hx, hy, hz);
+        const float leaf_cost = kCostIsect * (float)n * A;
+        const float split_cost = kCostTrav * A + kCostIsect * best;
+        int kind;   // 0 leaf, 1 SAH split, 2 median split
+        if (n <= 1) kind = 0;
+        else if (n <= kMaxLeaf && (axis < 0 || leaf_cost <= split_cost)) kind = 0;
+        else if (axis < 0 || task.depth >= (uint32_t)(kMaxTreeDepth - 28)) kind = 2;
+        else kind = 1;
+        s_kind = kind;
+        s_axis = axis;
+        s_bin = axis >= 0 ? s_best_bin[axis] : -1;
+        if (kind == 1) {
+            uint32_t nl = 0;
+            for (int b = 0; b <= s_bin; b++) nl += s_cnt[axis][b];
+            s_nleft = nl;
+        } else {
+            s_nleft = n / 2;
+        }
+    }
+    __syncthreads();
+
+    const int kind = s_kind;
+    SplitOut so;
+    so.kind = kind == 0 ? 0u : 1u;
+    so.nleft = s_nleft;
+    so.pad0 = so.pad1 = 0;
+    if (kind == 1) {
+        const int ax = s_axis, sb = s_bin;
+        // stable partition of idx[begin, end) into tmp, then copy back
+        const int PER = 4;
+        uint32_t lbase = task.begin, rbase = task.begin + s_nleft;
+        for (uint32_t t0 = 0; t0 < n; t0 += kBuildBlock * PER) {
+            uint32_t vals[PER];
+            uint32_t flags = 0, nl = 0;
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const uint32_t i = t0 + threadIdx.x * PER + e;
+                vals[e] = 0;
+                if (i < n) {
+                    const uint32_t p = idx[task.begin + i];
+                    vals[e] = p;
+                    const float4 a = plo[p], h = phi[p];
+                    const float c = .5f * (comp4(a, ax) + comp4(h, ax));
+                    if (bin_of(c, cl[ax], scale[ax]) <= sb) { flags |= 1u << e; nl++; }
+                }
+            }
+            uint32_t tot_l;
+            const uint32_t pre_l = block_scan<kBuildBlock>(nl, s_ws, tot_l);
+            const uint32_t cnt_here = (uint32_t)max(0, min(PER, (int)n - (int)(t0 + threadIdx.x * PER)));
+            const uint32_t pre_all = min(n - t0, threadIdx.x * (uint32_t)PER);
+            uint32_t l = lbase + pre_l, r = rbase + (pre_all - pre_l);
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                if ((uint32_t)e >= cnt_here) break;
+                if (flags & (1u << e)) tmp[l++] = vals[e]; else tmp[r++] = vals[e];
+            }
+            const uint32_t tile_n = min((uint32_t)(kBuildBlock * PER), n - t0);
+            lbase += tot_l;
+            rbase += tile_n - tot_l;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kBuildBlock) idx[task.begin + i] = tmp[task.begin + i];
+        if (threadIdx.x == 0) {
+            float l[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+            float r[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (int b = 0; b < kBins; b++) {
+                if (!s_cnt[ax][b]) continue;
+                float* d = b <= sb ? l : r;
+                for (int k = 0; k < 3; k++) {
+                    d[k] = fminf(d[k], o2f(s_b[ax][b][6 + k]));
+                    d[3 + k] = fmaxf(d[3 + k], o2f(s_b[ax][b][9 + k]));
+                }
+            }
+            so.clo_l = make_float4(l[0], l[1], l[2], 0.f);
+            so.chi_l = make_float4(l[3], l[4], l[5], 0.f);
+            so.clo_r = make_float4(r[0], r[1], r[2], 0.f);
+            so.chi_r = make_float4(r[3], r[4], r[5], 0.f);
+            out[blockIdx.x] = so;
+        }
+    } else if (threadIdx.x == 0) {
+        // leaf, or median split by index (children inherit the centroid bounds)
+        so.clo_l = so.clo_r = task.clo;
+        so.chi_l = so.chi_r = task.chi;
+        out[blockIdx.x] = so;
+    }
+}
+
+// single-block exclusive scan of `n` uints (kind flags) -> out, total -> *total
+__global__ __launch_bounds__(kScanBlock) void k_scan_kind(const SplitOut* __restrict__ so, uint32_t n,
+                                                         uint32_t* __restrict__ out, uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_ws[kScanBlock / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? so[i].kind : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan<kScanBlock>(v, s_ws, tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_emit(const BuildTask* __restrict__ tasks, const SplitOut* __restrict__ so,
+                       const uint32_t* __restrict__ rank, uint32_t ntasks, uint32_t node_base,
+                       BNode* __restrict__ bnodes, BuildTask* __restrict__ next,
+                       BuildStats* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntasks) return;
+    const BuildTask t = tasks[i];
+    const SplitOut s = so[i];
+    const uint32_t n = t.end - t.begin;
+    if (s.kind == 0) {
+        bnodes[t.node].lo.w = __uint_as_float(t.begin);
+        bnodes[t.node].hi.w = __uint_as_float(n);
+        atomicAdd(&st->leaves, 1u);
+        atomicMax(&st->max_depth, t.depth + 1);
+        atomicMax(&st->max_leaf, n);
+    } else {
+        const uint32_t L = node_base + 2 * rank[i];
+        bnodes[t.node].lo.w = __uint_as_float(L);
+        bnodes[t.node].hi.w = __uint_as_float(0u);
+        BuildTask a, b;
+        a.node = L; a.begin = t.begin; a.end = t.begin + s.nleft; a.depth = t.depth + 1;
+        a.clo = s.clo_l; a.chi = s.chi_l;
+        b.node = L + 1; b.begin = t.begin + s.nleft; b.end = t.end; b.depth = t.depth + 1;
+        b.clo = s.clo_r; b.chi = s.chi_r;
+        next[2 * rank[i]] = a;
+        next[2 * rank[i] + 1] = b;
+    }
+}
+
+// SAH cost of the finished tree (relative to the root area)
+__global__ void k_sah(const BNode* __restrict__ bnodes, uint32_t nnodes, BuildStats* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < nnodes) {
+        const BNode nd = bnodes[i];
+        const float a = area3(nd.lo.x, nd.lo.y, nd.lo.z, nd.hi.x, nd.hi.y, nd.hi.z);
+        const uint32_t cnt = __float_as_uint(nd.hi.w);
+        v = cnt ? (double)kCostIsect * cnt * a : (double)kCostTrav * a;
+    }
+    // wave reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&st->sah, v);
+}
+
+__global__ void k_inner_flag(const BNode* __restrict__ bnodes, uint32_t nnodes, SplitOut* __restrict__ flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnodes) flag[i].kind = __float_as_uint(bnodes[i].hi.w) == 0u ? 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t child_ref(const BNode& c, uint32_t cidx, const uint32_t* tidx) {
+    const uint32_t cnt = __float_as_uint(c.hi.w);
+    return cnt ? leaf_ref(__float_as_uint(c.lo.w), cnt) : tidx[cidx];
+}
+
+__global__ void k_convert(const BNode* __restrict__ bnodes, uint32_t nnodes, const uint32_t* __restrict__ tidx,
+                          float4* __restrict__ nodes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const BNode nd = bnodes[i];
+    if (__float_as_uint(nd.hi.w) != 0u) return;   // leaves have no traversal node
+    const uint32_t L = __float_as_uint(nd.lo.w);
+    const BNode c0 = bnodes[L], c1 = bnodes[L + 1];
+    float4* o = nodes + 4 * (size_t)tidx[i];
+    o[0] = make_float4(c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y);
+    o[1] = make_float4(c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y);
+    o[2] = make_float4(c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z);
+    o[3] = make_float4(__uint_as_float(child_ref(c0, L, tidx)), __uint_as_float(child_ref(c1, L + 1, tidx)), 0.f, 0.f);
+}
+
+__global__ void k_gather_tris(const float4* __restrict__ tri_world, const uint32_t* __restrict__ idx, uint32_t N,
+                              float4* __restrict__ tris) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const size_t p = idx[k];
+    tris[3 * (size_t)k + 0] = tri_world[3 * p + 0];
+    tris[3 * (size_t)k + 1] = tri_world[3 * p + 1];
+    tris[3 * (size_t)k + 2] = tri_world[3 * p + 2];
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    T* release() { T* q = p; p = nullptr; return q; }
+};
+
+#define RTX_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) { if (err) *err = std::string(#expr) + ": " + hipGetErrorString(e_); return e_; } \
+    } while (0)
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t flatten_scene(const InstBuild* d_insts, uint32_t ninst, const MeshRef* d_meshes, uint32_t N,
+                         float4* d_tri_world, float4* d_plo, float4* d_phi, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flatten, dim3(blocks_for(N, 256)), dim3(256), 0, s, d_insts, ninst, d_meshes, N,
+                       d_tri_world, d_plo, d_phi);
+    return hipGetLastError();
+}
+
+hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float4* d_phi, uint32_t N,
+                     hipStream_t s, BuiltBVH* outp, std::string* err) {
+    BuiltBVH out;
+    if (N == 0) {
+        out.root = kEmptyRoot;
+        *outp = out;
+        return hipSuccess;
+    }
+    DevBuf<uint32_t> idx, tmp, rank, total;
+    DevBuf<BuildTask> tasks_a, tasks_b;
+    DevBuf<SplitOut> so;
+    DevBuf<BNode> bnodes;
+    DevBuf<BuildStats> st;
+    const size_t max_nodes = 2 * (size_t)N;
+    RTX_TRY(idx.alloc(N));
+    RTX_TRY(tmp.alloc(N));
+    RTX_TRY(rank.alloc(max_nodes));
+    RTX_TRY(total.alloc(1));
+    RTX_TRY(tasks_a.alloc(N));
+    RTX_TRY(tasks_b.alloc(N));
+    RTX_TRY(so.alloc(max_nodes));
+    RTX_TRY(bnodes.alloc(max_nodes));
+    RTX_TRY(st.alloc(1));
+    RTX_TRY(hipMemsetAsync(st.p, 0, sizeof(BuildStats), s));
+
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(N, 256)), dim3(256), 0, s, idx.p, N);
+    hipLaunchKernelGGL(k_root_task, dim3(1), dim3(1024), 0, s, d_plo, d_phi, N, tasks_a.p, idx.p);
+    RTX_TRY(hipGetLastError());
+
+    uint32_t ntasks = 1, nnodes = 1;
+    BuildTask* cur = tasks_a.p;
+    BuildTask* nxt = tasks_b.p;
+    while (ntasks > 0) {
+        hipLaunchKernelGGL(k_level, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, tmp.p,
+                           bnodes.p, so.p);
+        hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, ntasks, rank.p, total.p);
+        hipLaunchKernelGGL(k_emit, dim3(blocks_for(ntasks, 256)), dim3(256), 0, s, cur, so.p, rank.p, ntasks,
+                           nnodes, bnodes.p, nxt, st.p);
+        RTX_TRY(hipGetLastError());
+        uint32_t nsplit = 0;
+        RTX_TRY(hipMemcpyAsync(&nsplit, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        RTX_TRY(hipStreamSynchronize(s));
+        nnodes += 2 * nsplit;
+        ntasks = 2 * nsplit;
+        std::swap(cur, nxt);
+    }
+
+    hipLaunchKernelGGL(k_sah, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, st.p);
+    // compact inner nodes into the traversal layout (BFS order of the build)
+    hipLaunchKernelGGL(k_inner_flag, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, so.p);
+    hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, nnodes, rank.p, total.p);
+    RTX_TRY(hipGetLastError());
+    uint32_t ninner = 0;
+    BuildStats hst;
+    RTX_TRY(hipMemcpyAsync(&ninner, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RTX_TRY(hipMemcpyAsync(&hst, st.p, sizeof(BuildStats), hipMemcpyDeviceToHost, s));
+    BNode root;
+    RTX_TRY(hipMemcpyAsync(&root, bnodes.p, sizeof(BNode), hipMemcpyDeviceToHost, s));
+    RTX_TRY(hipStreamSynchronize(s));
+
+    DevBuf<float4> nodes, tris;
+    RTX_TRY(nodes.alloc(4 * (size_t)(ninner ? ninner : 1)));
+    RTX_TRY(tris.alloc(3 * (size_t)N));
+    if (ninner)
+        hipLaunchKernelGGL(k_convert, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, rank.p,
+                           nodes.p);
+    hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(N, 256)), dim3(256), 0, s, d_tri_world, idx.p, N, tris.p);
+    RTX_TRY(hipGetLastError());
+    RTX_TRY(hipStreamSynchronize(s));
+
+    const uint32_t root_cnt = __builtin_bit_cast(uint32_t, root.hi.w);
+    const float rx = root.hi.x - root.lo.x, ry = root.hi.y - root.lo.y, rz = root.hi.z - root.lo.z;
+    const double root_area = 2.0 * ((double)rx * ry + (double)ry * rz + (double)rz * rx);
+    out.nodes = nodes.release();
+    out.tris = tris.release();
+    out.root = root_cnt ? leaf_ref(__builtin_bit_cast(uint32_t, root.lo.w), root_cnt) : 0u;
+    out.num_nodes = ninner;
+    out.num_leaves = hst.leaves;
+    out.max_depth = hst.max_depth;
+    out.max_leaf = hst.max_leaf;
+    out.sah = root_area > 0.0 ? hst.sah / root_area : 0.0;
+    *outp = out;
+    return hipSuccess;
+}
+
+}  // namespace rtx

@@ -1,0 +1,614 @@
+// capi.cpp -- the extern "C" boundary (include/rtx.h): context, geometry,
+// acceleration-structure build, launch, post-process and memory plumbing.
+//
+// Host-side counterparts: Scene (optx/scene.cxx), Launcher::ignite
+// (optx/launcher.cxx:49-77), Camera::set (optx/camera.h:30-48), Sphere
+// (optx/sphere.cxx) and pp_sRGB/pp_none (optx/postproc.cu:49-61).  Errors are
+// status codes plus a per-thread message (the reference throws
+// std::runtime_error from CUDA_CHECK/OPTX_CHECK, optx/util_cpu.h:18-53).
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bvh_build.h"
+#include "kernels.h"
+#include "rtx.h"
+#include "rtx_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+rtx_status fail(rtx_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+rtx_status hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? RTX_ERROR_OUT_OF_MEMORY : RTX_ERROR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_OK(expr)                                         \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);    \
+    } while (0)
+
+struct MeshRec {
+    float* d_v = nullptr;
+    uint32_t* d_i = nullptr;
+    uint32_t nv = 0, ni = 0;
+};
+
+struct ThingRec {
+    rtx_thing thing;
+    uint32_t mesh;
+    float xf[12];
+};
+
+struct Built {
+    rtx::DevScene hdr;                 // host copy of what the handle points at
+    rtx::DevScene* d_hdr = nullptr;    // device copy; its address is the handle
+    float4* nodes = nullptr;
+    float4* tris = nullptr;
+    float4* insts = nullptr;
+    float4* tri_world = nullptr;       // original-order world triangles (3 float4 each)
+    int device = 0;
+    rtx_scene_stats stats{};
+    void release() {
+        if (d_hdr) (void)hipFree(d_hdr);
+        if (nodes) (void)hipFree(nodes);
+        if (tris) (void)hipFree(tris);
+        if (insts) (void)hipFree(insts);
+        if (tri_world) (void)hipFree(tri_world);
+        d_hdr = nullptr; nodes = tris = insts = tri_world = nullptr;
+    }
+};
+
+// process-wide registry of live traversable handles (device address -> scene)
+std::mutex g_reg_mu;
+std::unordered_map<uint64_t, Built*> g_registry;
+
+Built* lookup(uint64_t handle) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto it = g_registry.find(handle);
+    return it == g_registry.end() ? nullptr : it->second;
+}
+
+const float kIdentity[12] = {1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+
+// device buffer freed at scope exit (build temporaries)
+template <typename T>
+struct Tmp {
+    T* p = nullptr;
+    ~Tmp() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct rtx_ctx {
+    int device = 0;
+    int num_cus = 0;
+    std::vector<MeshRec> meshes;
+    std::vector<ThingRec> things;
+    Built* built = nullptr;
+    rtx_float3 ambient{.5f, .7f, 1.f};
+    rtx::LaunchScratch render_ls, trace_ls;
+};
+
+extern "C" {
+
+const char* rtx_last_error(void) { return g_err.c_str(); }
+
+static rtx_status alloc_scratch(rtx::LaunchScratch& ls, uint32_t blocks_per_cu, int num_cus, int block) {
+    ls.grid = blocks_per_cu * (uint32_t)num_cus;
+    ls.gstride = ls.grid * (uint32_t)block;
+    HIP_OK(hipMalloc((void**)&ls.counter, sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void**)&ls.gstack,
+                     sizeof(uint32_t) * (size_t)(rtx::kMaxTreeDepth - rtx::kStackLDS) * ls.gstride));
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_ctx_create(int device, rtx_ctx** out) {
+    if (!out) return fail(RTX_ERROR_INVALID_VALUE, "rtx_ctx_create: ctx is NULL");
+    int n = 0;
+    HIP_OK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(RTX_ERROR_INVALID_VALUE, "rtx_ctx_create: no such device");
+    HIP_OK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, device));
+    rtx_ctx* c = new rtx_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    rtx_status st = alloc_scratch(c->render_ls, rtx::render_blocks_per_cu(), c->num_cus, rtx::kRenderBlock);
+    if (st == RTX_SUCCESS)
+        st = alloc_scratch(c->trace_ls, rtx::trace_blocks_per_cu(), c->num_cus, rtx::kTraceBlock);
+    if (st != RTX_SUCCESS) {
+        rtx_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return RTX_SUCCESS;
+}
+
+static void drop_built(rtx_ctx* c) {
+    if (!c->built) return;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        g_registry.erase((uint64_t)(uintptr_t)c->built->d_hdr);
+    }
+    c->built->release();
+    delete c->built;
+    c->built = nullptr;
+}
+
+rtx_status rtx_ctx_destroy(rtx_ctx* c) {
+    if (!c) return RTX_SUCCESS;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    drop_built(c);
+    for (auto& m : c->meshes) {
+        if (m.d_v) (void)hipFree(m.d_v);
+        if (m.d_i) (void)hipFree(m.d_i);
+    }
+    for (auto* ls : {&c->render_ls, &c->trace_ls}) {
+        if (ls->counter) (void)hipFree(ls->counter);
+        if (ls->gstack) (void)hipFree(ls->gstack);
+    }
+    delete c;
+    return RTX_SUCCESS;
+}
+
+// ---- camera: optx/camera.h:30-48 ------------------------------------------
+rtx_status rtx_camera_set(rtx_camera* c, rtx_float3 eye, rtx_float3 pat, rtx_float3 vup, float fov, float aspratio,
+                          float aperture, float fostance) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_camera_set: cam is NULL");
+    auto sub = [](rtx_float3 a, rtx_float3 b) { return rtx_float3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto scl = [](float s, rtx_float3 a) { return rtx_float3{s * a.x, s * a.y, s * a.z}; };
+    auto dot = [](rtx_float3 a, rtx_float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    auto cross = [](rtx_float3 a, rtx_float3 b) {
+        return rtx_float3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    };
+    auto unit = [&](rtx_float3 a) { return scl(1.f / std::sqrt(dot(a, a)), a); };
+    c->eye = eye; c->pat = pat; c->vup = vup;
+    c->fov = fov; c->aspratio = aspratio; c->aperture = aperture; c->fostance = fostance;
+    c->w = unit(sub(eye, pat));
+    c->u = unit(cross(vup, c->w));
+    c->v = cross(c->w, c->u);
+    const float kPi = 3.14159265358979323846f;
+    const float rad = fov * kPi / 180.f;
+    const float h = 2.f * std::tan(.5f * rad);
+    const float w = h * aspratio;
+    c->hvec = scl(fostance * h / 2.f, c->v);
+    c->wvec = scl(fostance * w / 2.f, c->u);
+    c->dvec = scl(fostance, c->w);
+    return RTX_SUCCESS;
+}
+
+// ---- sphere tessellation: optx/sphere.cxx:28-106 --------------------------
+namespace {
+struct V3 { float x, y, z; };
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 unitv(V3 a) { return (1.f / std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z)) * a; }
+
+// recursive 4-way split of a spherical triangle; leaves emit 3 scaled vertices
+void subdivide(std::vector<V3>& out, float radius, V3 a, V3 b, V3 c, uint32_t level) {
+    if (level == 0) {
+        out.push_back(radius * a);
+        out.push_back(radius * b);
+        out.push_back(radius * c);
+        return;
+    }
+    const V3 ab = unitv(.5f * (a + b)), bc = unitv(.5f * (b + c)), ca = unitv(.5f * (c + a));
+    subdivide(out, radius, a, ab, ca, level - 1);
+    subdivide(out, radius, ab, b, bc, level - 1);
+    subdivide(out, radius, ca, bc, c, level - 1);
+    subdivide(out, radius, ab, bc, ca, level - 1);
+}
+
+// exact-value key (+0 and -0 are one vertex, as the reference's comparator has it)
+struct KeyHash {
+    size_t operator()(const std::array<uint32_t, 3>& k) const {
+        return (size_t)k[0] * 0x9E3779B97F4A7C15ull ^ (size_t)k[1] * 0xC2B2AE3D27D4EB4Full ^ (size_t)k[2] * 0x165667B19E3779F9ull;
+    }
+};
+inline uint32_t key_bits(float f) {
+    if (f == 0.f) f = 0.f;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+}  // namespace
+
+rtx_status rtx_sphere_mesh(float radius, uint32_t ndiv, rtx_float3* vces, uint32_t vcap, uint32_t* nv,
+                           rtx_uint3* ices, uint32_t icap, uint32_t* ni) {
+    if (!nv || !ni) return fail(RTX_ERROR_INVALID_VALUE, "rtx_sphere_mesh: nv/ni is NULL");
+    if (ndiv > 11) return fail(RTX_ERROR_LIMIT, "rtx_sphere_mesh: ndiv > 11");
+    const uint64_t faces = 4ull << (2 * ndiv);
+    const uint64_t verts = 2ull * (1ull << (2 * ndiv)) + 2ull;
+    *nv = (uint32_t)verts;
+    *ni = (uint32_t)faces;
+    if (!vces && !ices) return RTX_SUCCESS;
+    if (!vces || !ices || vcap < verts || icap < faces)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_sphere_mesh: output capacity too small");
+    const float m = .57735026919f;   // optx/sphere.cxx:34
+    const V3 v0{m, m, m}, v1{m, -m, -m}, v2{-m, -m, m}, v3{-m, m, -m};
+    std::vector<V3> soup;
+    soup.reserve(3 * faces);
+    subdivide(soup, radius, v0, v1, v2, ndiv);
+    subdivide(soup, radius, v0, v2, v3, ndiv);
+    subdivide(soup, radius, v0, v3, v1, ndiv);
+    subdivide(soup, radius, v3, v2, v1, ndiv);
+    std::unordered_map<std::array<uint32_t, 3>, uint32_t, KeyHash> seen;
+    seen.reserve(2 * verts);
+    uint32_t count = 0;
+    std::vector<uint32_t> idx(soup.size());
+    for (size_t i = 0; i < soup.size(); i++) {
+        const V3 p = soup[i];
+        const std::array<uint32_t, 3> k{key_bits(p.x), key_bits(p.y), key_bits(p.z)};
+        auto it = seen.find(k);
+        if (it == seen.end()) {
+            if (count >= verts) return fail(RTX_ERROR_LIMIT, "rtx_sphere_mesh: vertex count mismatch");
+            seen.emplace(k, count);
+            vces[count] = rtx_float3{p.x, p.y, p.z};
+            idx[i] = count++;
+        } else {
+            idx[i] = it->second;
+        }
+    }
+    for (uint64_t f = 0; f < faces; f++) ices[f] = rtx_uint3{idx[3 * f], idx[3 * f + 1], idx[3 * f + 2]};
+    *nv = count;
+    return RTX_SUCCESS;
+}
+
+// ---- geometry / instances (Scene) ------------------------------------------
+rtx_status rtx_mesh_add(rtx_ctx* c, const rtx_float3* vces, uint32_t nv, const rtx_uint3* ices, uint32_t ni,
+                        uint32_t* mesh_id) {
+    if (!c || !mesh_id) return fail(RTX_ERROR_INVALID_VALUE, "rtx_mesh_add: NULL argument");
+    if ((nv && !vces) || (ni && !ices)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_mesh_add: NULL buffer");
+    for (uint32_t i = 0; i < ni; i++)
+        if (ices[i].x >= nv || ices[i].y >= nv || ices[i].z >= nv)
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_mesh_add: index out of bounds at triangle " + std::to_string(i));
+    HIP_OK(hipSetDevice(c->device));
+    MeshRec m;
+    m.nv = nv;
+    m.ni = ni;
+    HIP_OK(hipMalloc((void**)&m.d_v, sizeof(rtx_float3) * (nv ? nv : 1)));
+    HIP_OK(hipMalloc((void**)&m.d_i, sizeof(rtx_uint3) * (ni ? ni : 1)));
+    if (nv) HIP_OK(hipMemcpy(m.d_v, vces, sizeof(rtx_float3) * nv, hipMemcpyHostToDevice));
+    if (ni) HIP_OK(hipMemcpy(m.d_i, ices, sizeof(rtx_uint3) * ni, hipMemcpyHostToDevice));
+    *mesh_id = (uint32_t)c->meshes.size();
+    c->meshes.push_back(m);
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_thing_add(rtx_ctx* c, rtx_thing* thing, uint32_t mesh_id, uint32_t* thing_id) {
+    if (!c || !thing || !thing_id) return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_add: NULL argument");
+    if (mesh_id >= c->meshes.size()) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_thing_add: unknown mesh id");
+    if (thing->optics.type < 0 || thing->optics.type >= RTX_OPTICS_NUM)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_add: unknown optics type");
+    thing->vces = (rtx_float3*)c->meshes[mesh_id].d_v;    // optx/scene.cxx:191-192
+    thing->ices = (rtx_uint3*)c->meshes[mesh_id].d_i;
+    ThingRec r;
+    r.thing = *thing;
+    r.mesh = mesh_id;
+    std::memcpy(r.xf, kIdentity, sizeof(r.xf));
+    *thing_id = (uint32_t)c->things.size();
+    c->things.push_back(r);
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_thing_set(rtx_ctx* c, uint32_t id, const float transform[12]) {
+    if (!c || !transform) return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_set: NULL argument");
+    if (id >= c->things.size()) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_thing_set: id out of range");
+    std::memcpy(c->things[id].xf, transform, sizeof(float) * 12);
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_thing_get(rtx_ctx* c, uint32_t id, float transform[12]) {
+    if (!c || !transform) return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_get: NULL argument");
+    if (id >= c->things.size()) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_thing_get: id out of range");
+    std::memcpy(transform, c->things[id].xf, sizeof(float) * 12);
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_thing_at(rtx_ctx* c, uint32_t id, rtx_thing* thing) {
+    if (!c || !thing) return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_at: NULL argument");
+    if (id >= c->things.size()) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_thing_at: id out of range");
+    *thing = c->things[id].thing;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_thing_count(rtx_ctx* c, uint32_t* n) {
+    if (!c || !n) return fail(RTX_ERROR_INVALID_VALUE, "rtx_thing_count: NULL argument");
+    *n = (uint32_t)c->things.size();
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_set_miss_color(rtx_ctx* c, rtx_float3 ambient) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_set_miss_color: ctx is NULL");
+    c->ambient = ambient;
+    return RTX_SUCCESS;
+}
+
+// Flattens every instance and builds the BVH into a new Built (not registered).
+static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
+    HIP_OK(hipSetDevice(c->device));
+    const uint32_t ninst = (uint32_t)c->things.size();
+    std::vector<rtx::InstBuild> ib(ninst ? ninst : 1);
+    std::vector<float4> irec(2 * (size_t)(ninst ? ninst : 1));
+    uint64_t N = 0;
+    for (uint32_t i = 0; i < ninst; i++) {
+        const ThingRec& t = c->things[i];
+        std::memcpy(ib[i].xf, t.xf, sizeof(t.xf));
+        ib[i].mesh = t.mesh;
+        ib[i].tri_base = (uint32_t)N;
+        ib[i].pad[0] = ib[i].pad[1] = 0;
+        N += c->meshes[t.mesh].ni;
+        const rtx_optics& op = t.thing.optics;
+        float e = 0.f;
+        rtx_float3 alb{0.f, 0.f, 0.f};
+        if (op.type == RTX_OPTICS_DIFFUSE) alb = op.diffuse.albedo;
+        else if (op.type == RTX_OPTICS_REFLECT) { alb = op.reflect.albedo; e = op.reflect.fuzz; }
+        else e = op.refract.index;
+        int32_t ty = op.type;
+        float tyf;
+        std::memcpy(&tyf, &ty, 4);
+        irec[2 * i] = make_float4(alb.x, alb.y, alb.z, e);
+        irec[2 * i + 1] = make_float4(t.xf[3], t.xf[7], t.xf[11], tyf);   // center: optx/optics_r.cu:376-380
+    }
+    if (N > rtx::kMaxPrims) return fail(RTX_ERROR_LIMIT, "rtx_scene_build: more than 2^27-1 triangles");
+
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, s));
+
+    Built* b = new Built();
+    b->device = c->device;
+    auto bail = [&](rtx_status st) { b->release(); delete b; (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; };
+    std::vector<rtx::MeshRef> mr(c->meshes.size() ? c->meshes.size() : 1);
+    for (size_t m = 0; m < c->meshes.size(); m++)
+        mr[m] = rtx::MeshRef{c->meshes[m].d_v, c->meshes[m].d_i, c->meshes[m].nv, c->meshes[m].ni};
+    Tmp<rtx::InstBuild> d_ib;
+    Tmp<rtx::MeshRef> d_mr;
+    Tmp<float4> d_plo, d_phi;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&d_ib.p, sizeof(rtx::InstBuild) * ib.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(insts)"));
+    if ((e = hipMalloc((void**)&d_mr.p, sizeof(rtx::MeshRef) * mr.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(meshes)"));
+    if ((e = hipMalloc((void**)&b->insts, sizeof(float4) * irec.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(inst records)"));
+    if ((e = hipMalloc((void**)&b->tri_world, sizeof(float4) * 3 * (N ? N : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(triangles)"));
+    if ((e = hipMalloc((void**)&d_plo.p, sizeof(float4) * (N ? N : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(bounds)"));
+    if ((e = hipMalloc((void**)&d_phi.p, sizeof(float4) * (N ? N : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(bounds)"));
+    if ((e = hipMemcpyAsync(d_ib.p, ib.data(), sizeof(rtx::InstBuild) * ib.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload insts"));
+    if ((e = hipMemcpyAsync(d_mr.p, mr.data(), sizeof(rtx::MeshRef) * mr.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload meshes"));
+    if ((e = hipMemcpyAsync(b->insts, irec.data(), sizeof(float4) * irec.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload inst records"));
+    if ((e = rtx::flatten_scene(d_ib.p, ninst, d_mr.p, (uint32_t)N, b->tri_world, d_plo.p, d_phi.p, s)) != hipSuccess) return bail(hip_fail(e, "flatten"));
+    rtx::BuiltBVH bvh;
+    std::string err;
+    if ((e = rtx::build_bvh(b->tri_world, d_plo.p, d_phi.p, (uint32_t)N, s, &bvh, &err)) != hipSuccess)
+        return bail(fail(e == hipErrorOutOfMemory ? RTX_ERROR_OUT_OF_MEMORY : RTX_ERROR_HIP, "build_bvh: " + err));
+    b->nodes = bvh.nodes;
+    b->tris = bvh.tris;
+    b->hdr.nodes = bvh.nodes;
+    b->hdr.tris = bvh.tris;
+    b->hdr.insts = b->insts;
+    b->hdr.root = bvh.root;
+    b->hdr.num_prims = (uint32_t)N;
+    b->hdr.num_nodes = bvh.num_nodes;
+    b->hdr.num_insts = ninst;
+    if ((e = hipMalloc((void**)&b->d_hdr, sizeof(rtx::DevScene))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(handle)"));
+    if ((e = hipMemcpyAsync(b->d_hdr, &b->hdr, sizeof(rtx::DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload handle"));
+    if ((e = hipEventRecord(e1, s)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bvh.max_depth > (uint32_t)rtx::kMaxTreeDepth) {
+        b->release();
+        delete b;
+        return fail(RTX_ERROR_LIMIT, "rtx_scene_build: tree deeper than the traversal stack");
+    }
+    b->stats.num_prims = (uint32_t)N;
+    b->stats.num_nodes = bvh.num_nodes;
+    b->stats.num_leaves = bvh.num_leaves;
+    b->stats.max_depth = bvh.max_depth;
+    b->stats.max_leaf = bvh.max_leaf;
+    b->stats.sah_cost = (float)bvh.sah;
+    b->stats.build_ms = ms;
+    b->stats.device_bytes = sizeof(float4) * (4ull * bvh.num_nodes + 3ull * N + 2ull * ninst);
+    *out = b;
+    return RTX_SUCCESS;
+}
+
+// Scene::build, optx/scene.cxx:227-273 (the previous build of this context is freed, as Scene::free does)
+rtx_status rtx_scene_build(rtx_ctx* c, void* stream, uint64_t* is_handle) {
+    if (!c || !is_handle) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_build: NULL argument");
+    Built* b = nullptr;
+    rtx_status st = build_scene(c, (hipStream_t)stream, &b);
+    if (st != RTX_SUCCESS) return st;
+    drop_built(c);
+    c->built = b;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        g_registry[(uint64_t)(uintptr_t)b->d_hdr] = b;
+    }
+    *is_handle = (uint64_t)(uintptr_t)b->d_hdr;
+    return RTX_SUCCESS;
+}
+
+// Scene::update, optx/scene.cxx:275-294: brings the structure up to date with
+// the current transforms (here a rebuild) and keeps the handle: the new scene
+// header is written into the handle's device allocation.
+rtx_status rtx_scene_update(rtx_ctx* c, uint64_t is_handle, void* stream) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_update: ctx is NULL");
+    if (!c->built || (uint64_t)(uintptr_t)c->built->d_hdr != is_handle)
+        return fail(RTX_ERROR_INVALID_HANDLE, "rtx_scene_update: handle is not this context's scene");
+    hipStream_t s = (hipStream_t)stream;
+    Built* b = nullptr;
+    rtx_status st = build_scene(c, s, &b);
+    if (st != RTX_SUCCESS) return st;
+    Built* old = c->built;
+    HIP_OK(hipMemcpyAsync(old->d_hdr, &b->hdr, sizeof(rtx::DevScene), hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    (void)hipFree(b->d_hdr);
+    b->d_hdr = old->d_hdr;
+    old->d_hdr = nullptr;
+    old->release();
+    delete old;
+    c->built = b;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        g_registry[is_handle] = b;
+    }
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_scene_stats_get(rtx_ctx* c, uint64_t is_handle, rtx_scene_stats* st) {
+    if (!c || !st) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_stats_get: NULL argument");
+    Built* b = lookup(is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_scene_stats_get: unknown handle");
+    *st = b->stats;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_scene_triangles(rtx_ctx* c, uint64_t is_handle, float* tris, uint32_t* tri_inst) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_triangles: ctx is NULL");
+    Built* b = lookup(is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_scene_triangles: unknown handle");
+    const uint32_t N = b->hdr.num_prims;
+    if (N == 0) return RTX_SUCCESS;
+    HIP_OK(hipSetDevice(b->device));
+    std::vector<float4> h(3 * (size_t)N);
+    HIP_OK(hipMemcpy(h.data(), b->tri_world, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < N; p++) {
+        for (int k = 0; k < 3; k++) {
+            const float4 v = h[3 * p + k];
+            if (tris) { tris[9 * p + 3 * k] = v.x; tris[9 * p + 3 * k + 1] = v.y; tris[9 * p + 3 * k + 2] = v.z; }
+        }
+        if (tri_inst) std::memcpy(&tri_inst[p], &h[3 * p + 1].w, 4);
+    }
+    return RTX_SUCCESS;
+}
+
+// ---- launch: Launcher::ignite, optx/launcher.cxx:49-77 ---------------------
+rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stream) {
+    if (!c || !lp) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: NULL argument");
+    Built* b = lookup(lp->is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch: unknown is_handle");
+    if (b->device != c->device) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch: handle is on another device");
+    if (lp->image_w == 0 || lp->image_h == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: empty image");
+    if ((uint64_t)lp->image_w * lp->image_h > 0xFFFFFFFFull / 2)
+        return fail(RTX_ERROR_LIMIT, "rtx_launch: image too large");
+    HIP_OK(hipSetDevice(c->device));
+    const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
+    hipError_t e;
+    if (once || lp->picker) {
+        if (!lp->pick_id) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: picker without pick_id");
+        if (lp->pick_x >= lp->image_w || lp->pick_y >= lp->image_h)
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: pick position outside the image");
+        e = rtx::launch_pick(b->hdr, *lp, c->render_ls, (hipStream_t)stream);
+    } else {
+        if (lp->spp == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: spp == 0");
+        if (lp->depth == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: depth == 0");
+        if (!lp->rawRGB) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: rawRGB is NULL");
+        e = rtx::launch_render(b->hdr, *lp, amb, c->render_ls, (hipStream_t)stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "rtx_launch");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* node_visits, uint64_t* tri_tests,
+                              void* stream) {
+    if (!c || !lp || !node_visits || !tri_tests) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: NULL argument");
+    Built* b = lookup(lp->is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch_counted: unknown is_handle");
+    if (lp->image_w == 0 || lp->image_h == 0 || lp->spp == 0 || lp->depth == 0 || !lp->rawRGB)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: bad launch parameters");
+    HIP_OK(hipSetDevice(c->device));
+    rtx::TraceCounters tc;
+    tc.nodes = node_visits;
+    tc.tris = tri_tests;
+    const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
+    hipError_t e = rtx::launch_render_counted(b->hdr, *lp, amb, c->render_ls, tc, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_launch_counted");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_trace(rtx_ctx* c, uint64_t is_handle, const rtx_ray* rays, uint32_t n, rtx_hit* hits, void* stream) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_trace: ctx is NULL");
+    Built* b = lookup(is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_trace: unknown handle");
+    if (n && (!rays || !hits)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_trace: NULL buffer");
+    HIP_OK(hipSetDevice(c->device));
+    hipError_t e = rtx::launch_trace(b->hdr, rays, n, hits, c->trace_ls, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_trace");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_rng_draws(const uint32_t* seeds, uint32_t n, uint32_t k, float* out, void* stream) {
+    if (n && k && (!seeds || !out)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_rng_draws: NULL buffer");
+    hipError_t e = rtx::launch_rng(seeds, n, k, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_rng_draws");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_pp_srgb(const rtx_float3* src, rtx_uchar4* dst, int w, int h, void* stream) {
+    if (w < 0 || h < 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_pp_srgb: negative size");
+    if ((w && h) && (!src || !dst)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_pp_srgb: NULL buffer");
+    hipError_t e = rtx::launch_pp(src, dst, w, h, true, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_pp_srgb");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_pp_none(const rtx_float3* src, rtx_uchar4* dst, int w, int h, void* stream) {
+    if (w < 0 || h < 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_pp_none: negative size");
+    if ((w && h) && (!src || !dst)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_pp_none: NULL buffer");
+    hipError_t e = rtx::launch_pp(src, dst, w, h, false, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_pp_none");
+    return RTX_SUCCESS;
+}
+
+// ---- plumbing ---------------------------------------------------------------
+rtx_status rtx_malloc(void** p, size_t bytes) {
+    if (!p) return fail(RTX_ERROR_INVALID_VALUE, "rtx_malloc: NULL argument");
+    HIP_OK(hipMalloc(p, bytes ? bytes : 1));
+    return RTX_SUCCESS;
+}
+rtx_status rtx_free(void* p) {
+    if (p) HIP_OK(hipFree(p));
+    return RTX_SUCCESS;
+}
+rtx_status rtx_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RTX_SUCCESS;
+}
+rtx_status rtx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RTX_SUCCESS;
+}
+rtx_status rtx_memset(void* p, int value, size_t bytes) {
+    if (bytes) HIP_OK(hipMemset(p, value, bytes));
+    return RTX_SUCCESS;
+}
+rtx_status rtx_synchronize(void) {
+    HIP_OK(hipDeviceSynchronize());
+    return RTX_SUCCESS;
+}
+rtx_status rtx_device_count(int* n) {
+    if (!n) return fail(RTX_ERROR_INVALID_VALUE, "rtx_device_count: NULL argument");
+    if (hipGetDeviceCount(n) != hipSuccess) *n = 0;
+    return RTX_SUCCESS;
+}
+
+}  // extern "C"

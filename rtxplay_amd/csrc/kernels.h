@@ -1,0 +1,43 @@
+// kernels.h -- host-side launchers of the render, query and post-process kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtx.h"
+#include "rtx_internal.h"
+
+namespace rtx {
+
+constexpr int kRenderBlock = 256;
+constexpr int kTraceBlock = 256;
+
+// per-context scratch for the persistent kernels
+struct LaunchScratch {
+    uint32_t* counter = nullptr;   // work counter (device)
+    uint32_t* gstack = nullptr;    // traversal stack overflow (device)
+    uint32_t gstride = 0;          // lanes covered by gstack (grid * block)
+    uint32_t grid = 0;             // persistent grid size in blocks
+};
+
+// Optional per-pixel traversal statistics (node visits, triangle tests).
+struct TraceCounters {
+    uint64_t* nodes = nullptr;
+    uint64_t* tris = nullptr;
+};
+
+hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,
+                         hipStream_t s);
+hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, float3 ambient,
+                                 const LaunchScratch& ls, const TraceCounters& tc, hipStream_t s);
+hipError_t launch_pick(const DevScene& sc, const rtx_lp_general& lp, const LaunchScratch& ls, hipStream_t s);
+hipError_t launch_trace(const DevScene& sc, const rtx_ray* rays, uint32_t n, rtx_hit* hits,
+                        const LaunchScratch& ls, hipStream_t s);
+hipError_t launch_rng(const uint32_t* seeds, uint32_t n, uint32_t k, float* out, hipStream_t s);
+hipError_t launch_pp(const rtx_float3* src, rtx_uchar4* dst, int w, int h, bool srgb, hipStream_t s);
+
+// occupancy-derived persistent grid (blocks) for the render / trace kernels
+uint32_t render_blocks_per_cu();
+uint32_t trace_blocks_per_cu();
+
+}  // namespace rtx

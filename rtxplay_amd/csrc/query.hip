@@ -1,0 +1,114 @@
+// query.hip -- batch closest-hit query, device RNG draws and the post-process
+// kernels.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "traverse.h"
+
+namespace rtx {
+
+namespace {
+
+// closest hit for a batch of rays (the optixTrace query on its own),
+// persistent grid with a grid-stride loop so that the stack overflow area is
+// sized by the grid, not by the batch
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(const DevScene sc, const rtx_ray* __restrict__ rays, uint32_t n,
+                                                 rtx_hit* __restrict__ hits, uint32_t* gstack, uint32_t gstride) {
+    __shared__ uint32_t s_stack[kStackLDS * BLOCK];
+    LaneStack<BLOCK> stk;
+    stk.lds = s_stack + threadIdx.x;
+    stk.glob = gstack + blockIdx.x * BLOCK + threadIdx.x;
+    stk.gstride = gstride;
+    stk.sp = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const rtx_ray ray = rays[i];
+        TRay r;
+        tray_init(r, f3(ray.ori.x, ray.ori.y, ray.ori.z), f3(ray.dir.x, ray.dir.y, ray.dir.z), ray.tmin);
+        HitFull h;
+        traverse<BLOCK>(sc, r, ray.tmax, h, stk);
+        rtx_hit o;
+        if (h.tri == kNoHit) {
+            o.t = __uint_as_float(0x7F800000u);
+            o.u = o.v = 0.f;
+            o.instance_id = o.primitive_index = o.prim = 0xFFFFFFFFu;
+        } else {
+            const float4* tp = sc.tris + 3 * (size_t)h.tri;
+            o.t = h.t;
+            o.u = h.u;
+            o.v = h.v;
+            o.instance_id = __float_as_uint(tp[1].w);
+            o.primitive_index = __float_as_uint(tp[2].w);
+            o.prim = h.prim;
+        }
+        o.pad[0] = o.pad[1] = 0u;
+        hits[i] = o;
+    }
+}
+
+__global__ void k_rng(const uint32_t* __restrict__ seeds, uint32_t n, uint32_t k, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Frand48 r;
+    r.init(seeds[i]);
+    for (uint32_t j = 0; j < k; j++) out[(size_t)i * k + j] = r.next();
+}
+
+// optx/postproc.cu:2-16: IEC 61966-2-1 curve, then x255 truncated to uchar
+__device__ __forceinline__ float srgb_curve(float x) {
+    return x < .0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - .055f;
+}
+__device__ __forceinline__ unsigned char to_u8(float c) {
+    const float v = c * 255.f;
+    return (unsigned char)(v >= 255.f ? 255u : (v > 0.f ? (uint32_t)v : 0u));
+}
+
+// one thread per pixel, pix = x + w*y (optx/postproc.cu:36-47); the reference
+// grid is 2D 32x32, this one 1D over the w*h pixels
+template <bool SRGB>
+__global__ void k_pp(const float* __restrict__ src, uchar4* __restrict__ dst, uint32_t npix) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const float r = src[3 * (size_t)i], g = src[3 * (size_t)i + 1], b = src[3 * (size_t)i + 2];
+    if (SRGB) dst[i] = make_uchar4(to_u8(srgb_curve(r)), to_u8(srgb_curve(g)), to_u8(srgb_curve(b)), 255);
+    else dst[i] = make_uchar4(to_u8(r), to_u8(g), to_u8(b), 255);
+}
+
+}  // namespace
+
+uint32_t trace_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<kTraceBlock>, kTraceBlock, 0) != hipSuccess ||
+        nb < 1)
+        nb = 1;
+    return (uint32_t)nb;
+}
+
+hipError_t launch_trace(const DevScene& sc, const rtx_ray* rays, uint32_t n, rtx_hit* hits, const LaunchScratch& ls,
+                        hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    if (grid > ls.grid) grid = ls.grid;
+    hipLaunchKernelGGL(k_trace<kTraceBlock>, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, n, hits, ls.gstack,
+                       ls.gstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng(const uint32_t* seeds, uint32_t n, uint32_t k, float* out, hipStream_t s) {
+    if (n == 0 || k == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rng, dim3((n + 255) / 256), dim3(256), 0, s, seeds, n, k, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pp(const rtx_float3* src, rtx_uchar4* dst, int w, int h, bool srgb, hipStream_t s) {
+    const uint32_t npix = (uint32_t)w * (uint32_t)h;
+    if (npix == 0) return hipSuccess;
+    const dim3 grid((npix + 255) / 256), block(256);
+    if (srgb)
+        hipLaunchKernelGGL(k_pp<true>, grid, block, 0, s, (const float*)src, (uchar4*)dst, npix);
+    else
+        hipLaunchKernelGGL(k_pp<false>, grid, block, 0, s, (const float*)src, (uchar4*)dst, npix);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

@@ -1,0 +1,113 @@
+// rtx_internal.h -- device data layouts and helpers shared by the kernels.
+//
+// Layout in HBM of a built scene (what a traversable handle points at):
+//   nodes : BVH2 traversal nodes, 64 B each = 4 x float4
+//           n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//           n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//           n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//           n3 = (ref c0, ref c1, -, -) as uint bits
+//           ref >= 0: inner node index; ref with bit 31 set: leaf,
+//           bits 27..30 = count-1, bits 0..26 = first leaf-ordered triangle.
+//   tris  : leaf-ordered triangles, 48 B each = 3 x float4
+//           (v0.xyz | global prim id), (v1.xyz | instance id),
+//           (v2.xyz | primitive index inside the instance's mesh)
+//   insts : 32 B per instance = 2 x float4
+//           (albedo.xyz | fuzz for metal, index for glass),
+//           (center.xyz = translation of the 3x4 transform | optics type)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtx.h"
+
+namespace rtx {
+
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr int kLeafCountShift = 27;
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+constexpr uint32_t kMaxLeaf = 8;          // triangles per leaf (<= 16 fits the encoding)
+constexpr uint32_t kMaxPrims = kLeafFirstMask;  // 134M triangles per scene
+constexpr int kMaxTreeDepth = 64;         // guaranteed by the builder
+constexpr int kStackLDS = 32;             // traversal stack entries per lane kept in LDS
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kEmptyRoot = 0xFFFFFFFFu;
+
+// What a traversable handle points at (device memory); kernels get a copy by value.
+struct DevScene {
+    const float4* nodes;
+    const float4* tris;
+    const float4* insts;
+    uint32_t root;        // inner node 0, a leaf ref, or kEmptyRoot
+    uint32_t num_prims;
+    uint32_t num_nodes;
+    uint32_t num_insts;
+};
+
+__host__ __device__ inline uint32_t leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafFlag | ((count - 1u) << kLeafCountShift) | first;
+}
+
+// ---- small float3 helpers (the V:: operators of optx/v.h:26-53) ------------
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float3 operator*(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float3 operator-(float3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// optx/v.h:51: 1/len * v
+__device__ __forceinline__ float3 unitv(float3 a) { return (1.f / sqrtf(dot(a, a))) * a; }
+__device__ __forceinline__ float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
+
+// ---- Frand48: optx/frand48.h:17-35 -----------------------------------------
+// Only the low 48 bits of the 64-bit LCG state reach the output, and they
+// depend only on the low 48 bits of the previous state, so the state is
+// carried modulo 2^48.  The float is RN(x) * 2^-48 with x < 2^48; RN(x) is
+// formed exactly as one correctly rounded add of two exact 24-bit halves.
+struct Frand48 {
+    uint64_t state;
+    __device__ __forceinline__ void init(uint32_t seed) {
+        state = seed;
+        for (int i = 0; i < 16; i++) (void)next();
+    }
+    __device__ __forceinline__ float next() {
+        state = (0x5DEECE66Dull * state + 0xBull) & 0xFFFFFFFFFFFFull;
+        const float hi = (float)(uint32_t)(state >> 24) * 0x1p-24f;     // exact
+        const float lo = (float)(uint32_t)(state & 0xFFFFFFu) * 0x1p-48f; // exact
+        return __fadd_rn(hi, lo);
+    }
+    // util::rnd(min,max): optx/util_gpu.h:28
+    __device__ __forceinline__ float range(float lo, float hi) { return lo + next() * (hi - lo); }
+};
+
+// optx/v.h:72 (draws x, then y, then z)
+__device__ __forceinline__ float3 rnd_in_sphere(Frand48& r) {
+    for (;;) {
+        const float x = r.range(-1.f, 1.f);
+        const float y = r.range(-1.f, 1.f);
+        const float z = r.range(-1.f, 1.f);
+        const float3 v = f3(x, y, z);
+        if (1.f > dot(v, v)) return v;
+    }
+}
+// optx/v.h:75 (draws x, then y)
+__device__ __forceinline__ float3 rnd_in_disk(Frand48& r) {
+    for (;;) {
+        const float x = r.range(-1.f, 1.f);
+        const float y = r.range(-1.f, 1.f);
+        const float3 v = f3(x, y, 0.f);
+        if (1.f > dot(v, v)) return v;
+    }
+}
+
+// ---- rays and hits ---------------------------------------------------------
+struct HitRec {
+    float t, u, v;
+    uint32_t tri;   // leaf-ordered triangle index, kNoHit on a miss
+};
+
+}  // namespace rtx

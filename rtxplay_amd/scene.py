@@ -1,0 +1,115 @@
+"""The RTOW final scene as RTWO::load builds it (optx/rtwo.cxx:137-244), drawn
+from one seeded Frand48 stream instead of glibc rand().
+
+Draw order (the reference leaves the albedo products' order to the compiler):
+per grid cell select, center x, center z; then for a diffuse sphere two RGB
+triples whose product is the albedo, for a metal sphere an RGB triple in
+[.5,1) and the fuzz in [0,.5), for glass nothing.  All arithmetic is FP32
+with one rounding per operation, as the C++ recipe does it.
+"""
+import numpy as np
+
+from .api import Scene, Sphere, Thing, TYPE_DIFFUSE, TYPE_REFLECT, TYPE_REFRACT
+
+F = np.float32
+_M48 = (1 << 48) - 1
+_TWO48 = F(2.0 ** 48)
+
+
+class Frand48:
+    """optx/frand48.h:17-35 on the host (float results bit-identical to the device)."""
+
+    def __init__(self, seed=4711):
+        self.state = int(seed) & 0xFFFFFFFFFFFFFFFF
+        for _ in range(16):
+            self()
+
+    def __call__(self):
+        self.state = (0x5DEECE66D * self.state + 0xB) & 0xFFFFFFFFFFFFFFFF
+        return F(self.state & _M48) / _TWO48
+
+    def range(self, lo, hi):
+        """util::rnd(min,max), optx/util_gpu.h:28"""
+        return F(lo) + self() * (F(hi) - F(lo))
+
+
+def sphere_things(seed=0, half=11):
+    """The sphere list: dicts with type, albedo, fuzz, index, center, radius, kind
+    (kind 0 ground, 1 small, 2 big), in instance order."""
+    rng = Frand48(seed)
+    out = [dict(type=TYPE_DIFFUSE, albedo=(F(.5), F(.5), F(.5)), fuzz=F(0), index=F(0),
+                center=(F(0), F(-1000), F(0)), radius=F(1000), kind=0)]
+    for a in range(-half, half):
+        for b in range(-half, half):
+            select = rng()
+            cx = F(a) + F(.9) * rng()
+            cz = F(b) + F(.9) * rng()
+            c = (cx, F(.2), cz)
+            dx, dy, dz = cx - F(4), F(.2) - F(.2), cz - F(0)
+            if np.sqrt(dx * dx + dy * dy + dz * dz) > F(.9):
+                if select < F(.8):
+                    r1, g1, b1 = rng(), rng(), rng()
+                    r2, g2, b2 = rng(), rng(), rng()
+                    out.append(dict(type=TYPE_DIFFUSE, albedo=(r1 * r2, g1 * g2, b1 * b2), fuzz=F(0), index=F(0),
+                                    center=c, radius=F(.2), kind=1))
+                elif select < F(.95):
+                    alb = (rng.range(.5, 1.), rng.range(.5, 1.), rng.range(.5, 1.))
+                    fz = rng.range(0., .5)
+                    out.append(dict(type=TYPE_REFLECT, albedo=alb, fuzz=fz, index=F(0), center=c, radius=F(.2),
+                                    kind=1))
+                else:
+                    out.append(dict(type=TYPE_REFRACT, albedo=(F(0), F(0), F(0)), fuzz=F(0), index=F(1.5),
+                                    center=c, radius=F(.2), kind=1))
+    out.append(dict(type=TYPE_REFRACT, albedo=(F(0),) * 3, fuzz=F(0), index=F(1.5), center=(F(0), F(1), F(0)),
+                    radius=F(1), kind=2))
+    out.append(dict(type=TYPE_DIFFUSE, albedo=(F(.4), F(.2), F(.1)), fuzz=F(0), index=F(0),
+                    center=(F(-4), F(1), F(0)), radius=F(1), kind=2))
+    out.append(dict(type=TYPE_REFLECT, albedo=(F(.7), F(.6), F(.5)), fuzz=F(0), index=F(0),
+                    center=(F(4), F(1), F(0)), radius=F(1), kind=2))
+    return out
+
+
+# the reference's own per-sphere tessellation levels (optx/rtwo.cxx:138-241)
+def mixed_level(thing, index, count):
+    if thing["kind"] == 0:
+        return 9
+    if thing["kind"] == 2:
+        return {TYPE_REFRACT: 8, TYPE_DIFFUSE: 6, TYPE_REFLECT: 3}[thing["type"]]
+    return 3 if thing["type"] == TYPE_REFRACT else 6
+
+
+def transform_for(thing):
+    r = thing["radius"]
+    c = thing["center"]
+    return np.array([r, 0, 0, c[0], 0, r, 0, c[1], 0, 0, r, c[2]], np.float32)
+
+
+class RTOWScene(Scene):
+    """RTWO::load (optx/rtwo.cxx:133-245) over unit-sphere meshes.
+
+    level: an int (every sphere at that subdivision level) or "mixed" (the
+    reference's per-sphere levels)."""
+
+    def __init__(self, seed=0, half=11, level=4, device=0):
+        super().__init__(device)
+        self.seed, self.half, self.level = seed, half, level
+        self.things_spec = []
+
+    def load(self):
+        spec = sphere_things(self.seed, self.half)
+        levels = [self.level if not isinstance(self.level, str) else mixed_level(t, i, len(spec))
+                  for i, t in enumerate(spec)]
+        meshes = {}
+        for lv in sorted(set(levels)):
+            meshes[lv] = self.add(Sphere(1.0, lv).mesh())
+        for t, lv in zip(spec, levels):
+            th = Thing(t["type"], albedo=t["albedo"], fuzz=t["fuzz"], index=t["index"])
+            tid = self.add(th, meshes[lv])
+            self.set(tid, transform_for(t))
+        self.things_spec = spec
+        self.levels = levels
+        return len(spec)
+
+
+# camera of optx/rtwo.cxx:100-108
+RTOW_CAMERA = dict(eye=(13., 2., 3.), pat=(0., 0., 0.), vup=(0., 1., 0.), fov=20., aperture=.1, fostance=10.)
