@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only the post-process kernel over 64M pixels (768 MiB read, 256 MiB written): "
+                         "a plain streaming kernel of known size that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (pixels would be sharded "
@@ -86,6 +89,15 @@ def main():
     level, w, h, spp, depth = WORKLOADS[args.workload]
     if rtx.device_count() < 1:
         sys.exit("bench.py needs a GPU")
+    if args.copy_probe:
+        n = 64 << 20
+        src, dst = rtx.DeviceBuffer(12 * n), rtx.DeviceBuffer(4 * n)
+        src.zero()
+        for _ in range(3):
+            rtx.pp_none(src.ptr, dst.ptr, 8192, n // 8192)
+        rtx.synchronize()
+        print(json.dumps({"copy_probe": {"pixels": n, "read_bytes": 12 * n, "write_bytes": 4 * n}}))
+        return
     t0 = time.perf_counter()
     scene = RTOWScene(seed=0, half=11, level=level)
     scene.load()

@@ -19,6 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librtx.so")
+APP = os.path.join(PKG, "rtwo")
 ARCH = os.environ.get("RTX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["bvh_build.hip", "render.hip", "query.hip"]
@@ -68,6 +69,9 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([cc, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + [j[0] for j in jobs])
     os.replace(tmp, LIB)
+    # the C++ batch driver (host only, plain g++ against the C-ABI)
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(PKG, "apps", "rtwo.cpp"), "-L" + PKG, "-lrtx", "-Wl,-rpath,$ORIGIN", "-o", APP])
     return LIB
 
 

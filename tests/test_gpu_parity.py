@@ -316,3 +316,36 @@ def test_postprocess_matches_oracle(rtx, orc):
     rtx.synchronize()
     assert np.array_equal(dst.download(np.uint8, (h, w, 4)), orc.pp_none(raw))
     src.free(); dst.free()
+
+
+# ---- the C++ batch driver over the same library ------------------------------------
+def test_cpp_rtwo_driver_matches_python_path(rtx):
+    """rtxplay_amd/rtwo (include/rtx/rtx.hpp over the C-ABI) renders the same
+    bytes as the Python mirror for the same seeded scene."""
+    import subprocess
+    from conftest import ROOT
+    from rtxplay_amd.scene import RTOWScene
+    exe = os.path.join(ROOT, "rtxplay_amd", "rtwo")
+    w, h, spp, depth = 48, 27, 3, 16
+    say("rtwo driver", w, h, spp)
+    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", str(spp), "-d", str(depth), "-l", "2", "-A", "RPP", "-S"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    tok = p.stdout.split()
+    assert tok[0] == "P3" and int(tok[1]) == w and int(tok[2]) == h
+    ppm = np.array(tok[4:4 + 3 * w * h], np.int64).reshape(h, w, 3)[::-1]
+    k = 4 + 3 * w * h
+    assert tok[k] == "P2"
+    pgm = np.array(tok[k + 4:k + 4 + w * h], np.int64).reshape(h, w)[::-1]
+    assert "(pixel, rays, milliseconds)" in p.stderr
+    s = RTOWScene(seed=0, half=11, level=2)
+    s.load()
+    s.build()
+    L, lp = _launch(rtx, s, w, h, spp, depth)
+    out = rtx.DeviceBuffer(4 * w * h)
+    rtx.pp_sRGB(lp.rawRGB, out.ptr, w, h)
+    rtx.synchronize()
+    img = out.download(np.uint8, (h, w, 4))
+    assert np.array_equal(ppm, img[..., :3].astype(np.int64))
+    assert np.array_equal(pgm, L.rays_per_pixel().astype(np.int64))
+    out.free(); L.free(); s.close()
