@@ -67,7 +67,7 @@ def lib():
             "orc_closest_f32_bvh": (None, [P, P, c_uint32, P, P, P, P, c_int]),
             "orc_decidable": (c_int, [c_double, c_double, c_double, c_double, c_double, c_double]),
             "orc_render": (c_uint64, [POINTER(SceneC), POINTER(CameraC), c_uint32, c_uint32, c_uint32, c_uint32,
-                                      c_uint32, c_uint32, c_int, P, P, P, P, P]),
+                                      c_uint32, c_uint32, c_int, P, P, P, P, P, c_uint32]),
             "orc_rtow_render": (c_uint64, [c_int, c_int, c_int, c_int, c_int, P]),
             "orc_rtow_scene": (c_uint32, [c_int, P, c_uint32]),
             "orc_pp_srgb": (None, [P, P, c_int, c_int]),
@@ -280,7 +280,7 @@ class Scene:
                         ctypes.cast(self.things, c_void_p).value, len(things), self.bvh.ptr,
                         (c_float * 3)(*ambient))
 
-    def render(self, cam, w, h, spp, depth, y0=0, y1=None, nthreads=8, want_sumsq=False):
+    def render(self, cam, w, h, spp, depth, y0=0, y1=None, nthreads=8, want_sumsq=False, seed_base=0):
         y1 = h if y1 is None else y1
         raw = np.zeros((h, w, 3), np.float32)
         rpp = np.zeros((h, w), np.uint32)
@@ -288,7 +288,8 @@ class Scene:
         alb = np.zeros((h, w, 3), np.float32)
         sq = np.zeros((h, w, 3), np.float32) if want_sumsq else None
         rays = lib().orc_render(ctypes.byref(self.c), ctypes.byref(cam), w, h, spp, depth, y0, y1, nthreads,
-                                _p(raw), _p(rpp), _p(nrm), _p(alb), _p(sq) if sq is not None else None)
+                                _p(raw), _p(rpp), _p(nrm), _p(alb), _p(sq) if sq is not None else None,
+                                seed_base)
         return dict(raw=raw, rpp=rpp, normals=nrm, albedos=alb, sumsq=sq, rays=rays)
 
 

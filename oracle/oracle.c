@@ -823,7 +823,7 @@ static f3 trace_rec(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
 
 typedef struct {
     const orc_scene* s; const orc_camera* cam;
-    uint32_t w, h, spp, depth, y0, y1;
+    uint32_t w, h, spp, depth, y0, y1, seed_base;
     float *raw, *normals, *albedos, *sumsq;
     uint32_t* rpp;
     uint64_t rays;
@@ -839,7 +839,7 @@ static void* render_worker(void* arg) {
         for (uint32_t x = 0; x < j->w; x++) {
             const uint32_t pix = j->w * y + x;          /* optx/camera_r.cu:35 */
             orc_frand48 rng;
-            orc_frand48_init(&rng, pix);                /* optx/camera_r.cu:42-43 */
+            orc_frand48_init(&rng, pix + j->seed_base); /* optx/camera_r.cu:42-43 */
             f3 color = mk(0, 0, 0), normal = mk(0, 0, 0), albedo = mk(0, 0, 0), sq = mk(0, 0, 0);
             uint32_t rpp = 0;
             for (uint32_t i = 0; i < j->spp; i++) {     /* optx/camera_r.cu:66-109 */
@@ -874,14 +874,15 @@ static void* render_worker(void* arg) {
 
 uint64_t orc_render(const orc_scene* s, const orc_camera* cam, uint32_t w, uint32_t h,
                     uint32_t spp, uint32_t depth, uint32_t y0, uint32_t y1, int nthreads,
-                    float* raw, uint32_t* rpp, float* normals, float* albedos, float* sumsq) {
+                    float* raw, uint32_t* rpp, float* normals, float* albedos, float* sumsq,
+                    uint32_t seed_base) {
     if (nthreads <= 0) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if (y1 > h) y1 = h;
     pthread_t th[256];
     rjob jobs[256];
     for (int i = 0; i < nthreads; i++) {
-        rjob jb = {s, cam, w, h, spp, depth, y0, y1, raw, normals, albedos, sumsq, rpp, 0, i, nthreads};
+        rjob jb = {s, cam, w, h, spp, depth, y0, y1, seed_base, raw, normals, albedos, sumsq, rpp, 0, i, nthreads};
         jobs[i] = jb;
     }
     for (int i = 1; i < nthreads; i++) pthread_create(&th[i], NULL, render_worker, &jobs[i]);

@@ -126,10 +126,13 @@ typedef struct {
 /* Renders rows [y0,y1) of a w x h image.  Outputs are full-size buffers
  * (only the row range is written): raw 3*w*h, rpp w*h, normals/albedos 3*w*h
  * (may be NULL), sumsq 3*w*h (per-pixel sum of squared sample colours, may be
- * NULL).  Returns total rays traced in the range. */
+ * NULL).  Pixel streams are seeded with pix + seed_base (0 = the reference's
+ * seeding; another base gives independent streams, e.g. for a high-spp
+ * reference image).  Returns total rays traced in the range. */
 uint64_t orc_render(const orc_scene* s, const orc_camera* cam, uint32_t w, uint32_t h,
                     uint32_t spp, uint32_t depth, uint32_t y0, uint32_t y1, int nthreads,
-                    float* raw, uint32_t* rpp, float* normals, float* albedos, float* sumsq);
+                    float* raw, uint32_t* rpp, float* normals, float* albedos, float* sumsq,
+                    uint32_t seed_base);
 
 /* ---- Restatement of rtow.cxx (FP64, analytic spheres, glibc rand) -------- */
 /* argorder: 0 = function arguments evaluated left to right, 1 = right to left

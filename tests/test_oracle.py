@@ -154,6 +154,15 @@ def test_sphere_mesh_radius(orc):
     assert np.allclose(np.linalg.norm(v, axis=1), 2.5, rtol=1e-6)
 
 
+def test_vectorised_transform_matches_c(orc):
+    """oracle.transform_points (numpy) == orc_transform_point (C), bit for bit."""
+    g = np.random.default_rng(4)
+    xf = g.normal(size=12).astype(np.float32)
+    v = g.normal(size=(200, 3)).astype(np.float32)
+    want = np.stack([orc.transform_point(xf, p) for p in v])
+    assert np.array_equal(orc.transform_points(xf, v).view(np.uint32), want.view(np.uint32))
+
+
 # ---- scene recipe -------------------------------------------------------------
 def test_scene_recipe_matches_host(orc):
     from rtxplay_amd.scene import sphere_things
