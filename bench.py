@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
+    ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only the post-process kernel over 64M pixels (768 MiB read, 256 MiB written): "
                          "a plain streaming kernel of known size that calibrates the HBM byte counters")
@@ -102,6 +103,8 @@ def main():
     scene = RTOWScene(seed=0, half=11, level=level)
     scene.load()
     handle = scene.build()
+    if args.shade_batch:
+        scene.tuning(shade_batch=args.shade_batch)
     st = scene.stats()
     log("scene: %d things, %d triangles, %d nodes, depth %d, SAH %.1f, build %.1f ms (host %.1f s)"
         % (len(scene.things_spec), st["num_prims"], st["num_nodes"], st["max_depth"], st["sah_cost"],

@@ -151,6 +151,23 @@ rtx_status rtx_scene_stats_get(rtx_ctx* ctx, uint64_t is_handle, rtx_scene_stats
  * memory.  Either pointer may be NULL. */
 rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, uint32_t* tri_inst);
 
+/* Launch tuning of a context (no reference counterpart).  shade_batch: in
+ * the render kernel a wave traverses until at least this many of its lanes
+ * hold a final closest hit, shades those and resumes the others (1..64; 64 =
+ * every lane of the wave finishes first; default 16).  inner_frac: a
+ * traversal round's node phase ends when fewer than inner_frac/64 of the
+ * traversing lanes are still on inner nodes; the lanes on a leaf then test
+ * it (1..64; 1 = every lane reaches a leaf first, 64 = the first lane that
+ * reaches a leaf ends the phase; default 36).  0 fields keep the current
+ * value. */
+typedef struct {
+    uint32_t shade_batch;
+    uint32_t inner_frac;
+    uint32_t reserved[6];
+} rtx_tuning;
+rtx_status rtx_tuning_get(rtx_ctx* ctx, rtx_tuning* t);
+rtx_status rtx_tuning_set(rtx_ctx* ctx, const rtx_tuning* t);
+
 /* Miss record (SBT miss data), optx/rtwo.cxx:110-111 and :474-489.  Default {.5,.7,1}. */
 rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
 
@@ -163,10 +180,15 @@ rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* st
 
 /* Statistics launch: the same render as rtx_launch, which also writes per
  * pixel the number of BVH node visits and triangle tests of all its rays
- * (device uint64 arrays of image_w*image_h).  No counterpart in the
- * reference; it feeds the bytes-per-ray model of the benchmark. */
+ * (device uint64 arrays of image_w*image_h) and, if wave_cycles is not NULL,
+ * adds to wave_cycles[0] / [1] the shader-clock cycles the waves spent in
+ * traversal / in the whole render loop and to [2] / [3] the wave-level
+ * iterations of the inner-node / triangle loops (4 device uint64, zeroed by
+ * the caller).  No
+ * counterpart in the reference; it feeds the bytes-per-ray model of the
+ * benchmark. */
 rtx_status rtx_launch_counted(rtx_ctx* ctx, const rtx_lp_general* lp, uint64_t* node_visits,
-                              uint64_t* tri_tests, void* stream);
+                              uint64_t* tri_tests, uint64_t* wave_cycles, void* stream);
 
 /* Closest-hit query for a batch of rays (the optixTrace call the programs
  * make, optx/camera_r.cu:82-99), for tests and hybrid callers.  rays and hits

@@ -89,6 +89,10 @@ class SceneStatsC(Structure):
                 ("build_ms", c_float), ("device_bytes", c_uint64)]
 
 
+class TuningC(Structure):
+    _fields_ = [("shade_batch", c_uint32), ("inner_frac", c_uint32), ("reserved", c_uint32 * 6)]
+
+
 # every exported symbol: name -> (restype, argtypes)
 SIGNATURES = {
     "rtx_last_error": (c_char_p, []),
@@ -108,8 +112,10 @@ SIGNATURES = {
     "rtx_scene_stats_get": (c_int, [c_void_p, c_uint64, POINTER(SceneStatsC)]),
     "rtx_scene_triangles": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_set_miss_color": (c_int, [c_void_p, Float3]),
+    "rtx_tuning_get": (c_int, [c_void_p, POINTER(TuningC)]),
+    "rtx_tuning_set": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_launch": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),
-    "rtx_launch_counted": (c_int, [c_void_p, POINTER(LpGeneralC), c_void_p, c_void_p, c_void_p]),
+    "rtx_launch_counted": (c_int, [c_void_p, POINTER(LpGeneralC), c_void_p, c_void_p, c_void_p, c_void_p]),
     "rtx_trace": (c_int, [c_void_p, c_uint64, c_void_p, c_uint32, c_void_p, c_void_p]),
     "rtx_rng_draws": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p]),
     "rtx_pp_srgb": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
@@ -166,5 +172,5 @@ def call(name, *args):
 
 
 __all__ = ["Float3", "Uint3", "Uchar4", "OpticsC", "ThingC", "CameraC", "LpGeneralC", "RayC", "HitC",
-           "SceneStatsC", "SIGNATURES", "RtxError", "lib", "check", "call", "byref", "c_void_p", "c_uint32",
+           "SceneStatsC", "TuningC", "SIGNATURES", "RtxError", "lib", "check", "call", "byref", "c_void_p", "c_uint32",
            "c_uint64", "c_float", "c_int"]

@@ -15,7 +15,7 @@ import ctypes
 
 import numpy as np
 
-from ._capi import (CameraC, Float3, HitC, LpGeneralC, OpticsC, RayC, RtxError, SceneStatsC, ThingC, byref,
+from ._capi import (CameraC, Float3, HitC, LpGeneralC, OpticsC, RayC, RtxError, SceneStatsC, ThingC, TuningC, byref,
                     c_float, c_uint32, c_uint64, c_void_p, call, lib)
 
 TYPE_DIFFUSE, TYPE_REFLECT, TYPE_REFRACT, TYPE_NUM = 0, 1, 2, 3
@@ -308,6 +308,16 @@ class Scene:
              inst.ctypes.data_as(c_void_p))
         return tris, inst
 
+    def tuning(self, **kw):
+        """Reads (and with keyword arguments sets) the launch tuning, e.g. shade_batch=32."""
+        t = TuningC()
+        call("rtx_tuning_get", self.ctx, byref(t))
+        if kw:
+            for k, v in kw.items():
+                setattr(t, k, int(v))
+            call("rtx_tuning_set", self.ctx, byref(t))
+        return {"shade_batch": t.shade_batch, "inner_frac": t.inner_frac}
+
     def set_miss_color(self, ambient):
         call("rtx_set_miss_color", self.ctx, _f3(ambient))
 
@@ -362,17 +372,21 @@ class Launcher:
         if sync:
             synchronize()
 
-    def ignite_counted(self, stream=None):
-        """Render once more with traversal counters: (node visits, triangle tests) per pixel."""
+    def ignite_counted(self, stream=None, cycles=False):
+        """Render once more with traversal counters: (node visits, triangle tests) per pixel
+        [, (traversal cycles, loop cycles, inner-node wave iterations, triangle wave iterations)]."""
         n = self.lp.image_w * self.lp.image_h
-        dn, dt = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
-        call("rtx_launch_counted", self.scene.ctx, byref(self.lp), dn.ptr, dt.ptr, _stream_ptr(stream))
+        dn, dt, dc = DeviceBuffer(8 * n), DeviceBuffer(8 * n), DeviceBuffer(32)
+        dc.zero()
+        call("rtx_launch_counted", self.scene.ctx, byref(self.lp), dn.ptr, dt.ptr, dc.ptr, _stream_ptr(stream))
         synchronize()
         shape = (self.lp.image_h, self.lp.image_w)
         out = dn.download(np.uint64, shape), dt.download(np.uint64, shape)
+        cyc = tuple(int(x) for x in dc.download(np.uint64, (4,)))
         dn.free()
         dt.free()
-        return out
+        dc.free()
+        return out + (cyc,) if cycles else out
 
     def free(self):
         for b in self._bufs.values():

@@ -334,6 +334,23 @@ rtx_status rtx_thing_count(rtx_ctx* c, uint32_t* n) {
     return RTX_SUCCESS;
 }
 
+rtx_status rtx_tuning_get(rtx_ctx* c, rtx_tuning* t) {
+    if (!c || !t) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_get: NULL argument");
+    std::memset(t, 0, sizeof(*t));
+    t->shade_batch = c->render_ls.shade_batch;
+    t->inner_frac = c->render_ls.inner_frac;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_tuning_set(rtx_ctx* c, const rtx_tuning* t) {
+    if (!c || !t) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: NULL argument");
+    if (t->shade_batch > 64) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: shade_batch > 64");
+    if (t->inner_frac > 64) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: inner_frac > 64");
+    if (t->shade_batch) c->render_ls.shade_batch = t->shade_batch;
+    if (t->inner_frac) c->render_ls.inner_frac = t->inner_frac;
+    return RTX_SUCCESS;
+}
+
 rtx_status rtx_set_miss_color(rtx_ctx* c, rtx_float3 ambient) {
     if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_set_miss_color: ctx is NULL");
     c->ambient = ambient;
@@ -529,7 +546,7 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
 }
 
 rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* node_visits, uint64_t* tri_tests,
-                              void* stream) {
+                              uint64_t* wave_cycles, void* stream) {
     if (!c || !lp || !node_visits || !tri_tests) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: NULL argument");
     Built* b = lookup(lp->is_handle);
     if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch_counted: unknown is_handle");
@@ -539,6 +556,7 @@ rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* no
     rtx::TraceCounters tc;
     tc.nodes = node_visits;
     tc.tris = tri_tests;
+    tc.wave_cycles = wave_cycles;
     const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
     hipError_t e = rtx::launch_render_counted(b->hdr, *lp, amb, c->render_ls, tc, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "rtx_launch_counted");

@@ -18,12 +18,15 @@ struct LaunchScratch {
     uint32_t* gstack = nullptr;    // traversal stack overflow (device)
     uint32_t gstride = 0;          // lanes covered by gstack (grid * block)
     uint32_t grid = 0;             // persistent grid size in blocks
+    uint32_t shade_batch = 16;     // render: finished lanes that trigger a wave's shading step
+    uint32_t inner_frac = 36;      // traversal: node phase ends below inner_frac/64 of the going lanes
 };
 
 // Optional per-pixel traversal statistics (node visits, triangle tests).
 struct TraceCounters {
     uint64_t* nodes = nullptr;
     uint64_t* tris = nullptr;
+    uint64_t* wave_cycles = nullptr;   // [0] cycles in traversal, [1] cycles in the whole loop (sum over waves)
 };
 
 hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,

@@ -17,10 +17,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const DevScene sc, const rtx_ra
                                                  rtx_hit* __restrict__ hits, uint32_t* gstack, uint32_t gstride) {
     __shared__ uint32_t s_stack[kStackLDS * BLOCK];
     LaneStack<BLOCK> stk;
-    stk.lds = s_stack + threadIdx.x;
+    stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = gstack + blockIdx.x * BLOCK + threadIdx.x;
     stk.gstride = gstride;
-    stk.sp = 0;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         const rtx_ray ray = rays[i];
         TRay r;

@@ -32,6 +32,8 @@ struct RenderArgs {
     uint32_t* gstack;
     uint32_t gstride;
     uint32_t tiles_x, tiles_y;
+    uint32_t batch;       // lanes that must finish a ray before a wave shades (64: whole wave)
+    uint32_t inner_frac;  // node phase ends below inner_frac/64 of the going lanes
     TraceCounters tc;
 };
 
@@ -46,13 +48,12 @@ __device__ __forceinline__ void st(rtx_float3* p, float3 v) {
 __global__ void k_zero(uint32_t* p) { *p = 0u; }
 
 template <int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_render(const RenderArgs a) {
     __shared__ uint32_t s_stack[kStackLDS * BLOCK];
     LaneStack<BLOCK> stk;
-    stk.lds = s_stack + threadIdx.x;
+    stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = a.gstack + blockIdx.x * BLOCK + threadIdx.x;
     stk.gstride = a.gstride;
-    stk.sp = 0;
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t W = a.lp.image_w, H = a.lp.image_h;
@@ -63,22 +64,27 @@ __global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
     const float3 hvec = ld(a.lp.camera.hvec), wvec = ld(a.lp.camera.wvec), dvec = ld(a.lp.camera.dvec);
     const float half_ap = a.lp.camera.aperture / 2.f;
 
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, going = false;
     uint32_t x = 0, y = 0, sample = 0, k = 0, rpp = 0;
     uint32_t cnt_nodes = 0, cnt_tris = 0;
     Frand48 rng;
     rng.state = 0;
-    float3 o = f3(0, 0, 0), d = f3(0, 0, 1), thr = f3(1, 1, 1);
+    float3 thr = f3(1, 1, 1);
     float3 acc = f3(0, 0, 0), nacc = f3(0, 0, 0), aacc = f3(0, 0, 0), gN = f3(0, 0, 0), gA = f3(0, 0, 0);
+    Trav tv;
+    trav_begin(tv, a.sc, f3(0, 0, 0), f3(0, 0, 1), 1e-3f, 1e16f);
+    uint64_t cyc_trav = 0;
+    uint32_t wave_iters[2] = {0, 0};
+    const uint64_t cyc_start = COUNT ? clock64() : 0;
 
-    // optx/camera_r.cu:69-73 and optx/camera.h:67-77
+    // optx/camera_r.cu:69-99 and optx/camera.h:67-77: the sample's primary ray
     auto start_sample = [&]() {
         const float s = 2.f * ((float)x + rng.next()) / (float)W - 1.f;
         const float t = 2.f * ((float)y + rng.next()) / (float)H - 1.f;
         const float3 rd = half_ap * rnd_in_disk(rng);
         const float3 off = rd.x * cu + rd.y * cv;
-        o = eye + off;
-        d = ((s * wvec + t * hvec) - dvec) - off;
+        trav_begin(tv, a.sc, eye + off, ((s * wvec + t * hvec) - dvec) - off, 1e-3f, 1e16f);
+        going = true;
         k = 0;
         thr = f3(1.f, 1.f, 1.f);
         gN = f3(0.f, 0.f, 0.f);
@@ -116,17 +122,20 @@ __global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
             }
         }
         if (!__any(active)) break;
-        if (!active) continue;
 
-        TRay r;
-        tray_init(r, o, d, 1e-3f);                         // tmin/tmax: optx/camera_r.cu:86-87
-        HitFull h;
-        traverse<BLOCK, COUNT>(a.sc, r, 1e16f, h, stk, &cnt_nodes, &cnt_tris);
+        // traverse until a batch of lanes has a final closest hit; the others
+        // keep their traversal state and resume after the shading below
+        const uint64_t c0 = COUNT ? clock64() : 0;
+        trav_run<BLOCK, COUNT>(a.sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+                               wave_iters);
+        if (COUNT) cyc_trav += clock64() - c0;
+        if (!active || going) continue;
+
         k++;                                               // payload 5 (rays per trace)
-
+        const float3 d = tv.r.d;
         bool end = false;
         float3 col = f3(0.f, 0.f, 0.f);
-        if (h.tri == kNoHit) {
+        if (tv.h.tri == kNoHit) {
             // __miss__ambient, optx/camera_r.cu:135-146
             const float3 unit = unitv(d);
             const float t = .5f * (unit.y + 1.f);
@@ -136,13 +145,13 @@ __global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
             end = true;                                    // optx/optics_r.cu:34-42
         } else {
             // hit reconstruction, optx/optics_r.cu:45-90
-            const float4* tp = a.sc.tris + 3 * (size_t)h.tri;
+            const float4* tp = a.sc.tris + 3 * (size_t)tv.h.tri;
             const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
             const uint32_t inst = __float_as_uint(t1.w);
             const float4 m0 = a.sc.insts[2 * inst], m1 = a.sc.insts[2 * inst + 1];
             const float3 A = xyz(t0), B = xyz(t1), C = xyz(t2);
-            const float w = 1.f - h.u - h.v;
-            const float3 hp = (w * A + h.u * B) + h.v * C;
+            const float w = 1.f - tv.h.u - tv.h.v;
+            const float3 hp = (w * A + tv.h.u * B) + tv.h.v * C;
             float3 N = unitv(cross(B - A, C - A));
             if (dot(d, N) > 0.f) N = -N;
             const int type = __float_as_int(m1.w);
@@ -184,8 +193,10 @@ __global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
                     nd = perpen + parall;
                 }
             }
-            o = hp;
-            d = nd;
+            if (!end) {
+                trav_begin(tv, a.sc, hp, nd, 1e-3f, 1e16f);   // optx/optics_r.cu:115-132
+                going = true;
+            }
         }
 
         if (end) {
@@ -210,6 +221,14 @@ __global__ __launch_bounds__(BLOCK) void k_render(const RenderArgs a) {
             }
         }
     }
+    if (COUNT && a.tc.wave_cycles) {
+        if (lane == 0) {
+            atomicAdd((unsigned long long*)&a.tc.wave_cycles[0], (unsigned long long)cyc_trav);
+            atomicAdd((unsigned long long*)&a.tc.wave_cycles[1], (unsigned long long)(clock64() - cyc_start));
+        }
+        atomicAdd((unsigned long long*)&a.tc.wave_cycles[2], (unsigned long long)wave_iters[0]);
+        atomicAdd((unsigned long long*)&a.tc.wave_cycles[3], (unsigned long long)wave_iters[1]);
+    }
 }
 
 // picker: one primary ray through (pick_x, pick_y), optx/camera_r.cu:26-32 and
@@ -218,10 +237,9 @@ __global__ __launch_bounds__(64) void k_pick(const RenderArgs a) {
     __shared__ uint32_t s_stack[kStackLDS * 64];
     if (threadIdx.x != 0) return;
     LaneStack<64> stk;
-    stk.lds = s_stack;
+    stk.lds = (lds_u32*)s_stack;
     stk.glob = a.gstack;
     stk.gstride = a.gstride;
-    stk.sp = 0;
     const uint32_t W = a.lp.image_w, H = a.lp.image_h;
     const uint32_t x = a.lp.pick_x, y = a.lp.pick_y;
     Frand48 rng;
@@ -249,6 +267,8 @@ RenderArgs make_args(const DevScene& sc, const rtx_lp_general& lp, float3 ambien
     a.gstride = ls.gstride;
     a.tiles_x = (lp.image_w + 7u) / 8u;
     a.tiles_y = (lp.image_h + 7u) / 8u;
+    a.batch = ls.shade_batch ? ls.shade_batch : 16u;
+    a.inner_frac = ls.inner_frac ? ls.inner_frac : 36u;
     return a;
 }
 

@@ -84,20 +84,26 @@ __device__ __forceinline__ bool watertight(const TRay& r, float3 v0, float3 v1, 
     return true;
 }
 
+// LDS addresses as such (address space 3), so that stack accesses compile to
+// ds_read/ds_write instead of flat loads through a generic pointer
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 template <int BLOCK>
 struct LaneStack {
-    uint32_t* lds;     // LDS column of this lane: entry i at lds[i * BLOCK]
+    lds_u32* lds;      // LDS column of this lane: entry i at lds[i * BLOCK]
     uint32_t* glob;    // overflow column: entry i at glob[(i - kStackLDS) * gstride]
     uint32_t gstride;
-    int sp;
-    __device__ __forceinline__ void push(uint32_t v) {
+    __device__ __forceinline__ void push(int& sp, uint32_t v) {
         if (sp < kStackLDS) lds[sp * BLOCK] = v;
         else glob[(size_t)(sp - kStackLDS) * gstride] = v;
         sp++;
     }
-    __device__ __forceinline__ uint32_t pop() {
+    __device__ __forceinline__ uint32_t pop(int& sp) {
         sp--;
-        return sp < kStackLDS ? lds[sp * BLOCK] : glob[(size_t)(sp - kStackLDS) * gstride];
+        uint32_t v;
+        if (sp < kStackLDS) v = lds[sp * BLOCK];
+        else v = glob[(size_t)(sp - kStackLDS) * gstride];
+        return v;
     }
 };
 
@@ -107,22 +113,63 @@ struct HitFull {
     uint32_t prim;   // global primitive id
 };
 
-// COUNT adds node-visit and triangle-test counts (statistics launches only)
+// Resumable traversal state of one ray: the ray, the next node to visit,
+// the stack depth and the best hit so far.
+struct Trav {
+    TRay r;
+    HitFull h;
+    uint32_t node;
+    int sp;
+};
+
+__device__ __forceinline__ void trav_begin(Trav& s, const DevScene& sc, float3 o, float3 d, float tmin, float tmax) {
+    tray_init(s.r, o, d, tmin);
+    s.h.t = tmax;
+    s.h.u = s.h.v = 0.f;
+    s.h.tri = kNoHit;
+    s.h.prim = kNoHit;
+    s.node = sc.root;
+    s.sp = 0;
+}
+
+// first active lane of the wave (counts wave-level loop iterations in COUNT mode)
+__device__ __forceinline__ bool first_active_lane() {
+    const uint64_t e = __builtin_amdgcn_read_exec();
+    return (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)e) - 1);
+}
+
+// Advances the traversal of the lanes with `going` set.  A lane clears
+// `going` when its ray is done (closest hit final in s.h).
+//
+// Each round has a node phase and a leaf phase.  The node phase steps the
+// lanes that sit on an inner node and ends when fewer than inner_frac/64 of
+// the going lanes still do (lanes that reached a leaf wait for it); the leaf
+// phase tests the triangles of every lane on a leaf.  inner_frac = 1 gives
+// the classic while-while loop (all lanes reach a leaf first).  The function
+// returns once no lane is going, or -- with `batch` < 64 -- as soon as at
+// least `batch` of the wave's `busy` lanes have finished, so that the caller
+// can shade them while the others keep their state and resume later.
+// COUNT adds node-visit and triangle-test counts (statistics launches only).
 template <int BLOCK, bool COUNT = false>
-__device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, float tmax, HitFull& h,
-                                         LaneStack<BLOCK>& st, uint32_t* nn = nullptr, uint32_t* nt = nullptr) {
-    h.t = tmax;
-    h.u = h.v = 0.f;
-    h.tri = kNoHit;
-    h.prim = kNoHit;
-    uint32_t node = sc.root;
-    if (node == kEmptyRoot) return;
-    st.sp = 0;
+__device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
+                                         int inner_frac, LaneStack<BLOCK>& st, uint32_t* nn = nullptr,
+                                         uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
+    if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
-        while (!(node & kLeafFlag)) {
-            if (COUNT) (*nn)++;
-            const float4* np = sc.nodes + 4 * (size_t)node;
+        // node phase
+        for (;;) {
+            const bool inner = going && !(s.node & kLeafFlag);
+            const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
+            if (n_inner == 0) break;
+            if (n_inner * 64u < (uint32_t)inner_frac * (uint32_t)__popcll(__ballot(going))) break;
+            if (!inner) continue;
+            if (COUNT) {
+                (*nn)++;
+                if (first_active_lane()) wi[0]++;
+            }
+            const float4* np = sc.nodes + 4 * (size_t)s.node;
             const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            const TRay& r = s.r;
             const float a0 = (n0.x - r.o.x) * r.idir.x, a1 = (n0.y - r.o.x) * r.idir.x;
             const float a2 = (n0.z - r.o.y) * r.idir.y, a3 = (n0.w - r.o.y) * r.idir.y;
             const float a4 = (n2.x - r.o.z) * r.idir.z, a5 = (n2.y - r.o.z) * r.idir.z;
@@ -130,46 +177,71 @@ __device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, floa
             const float b2 = (n1.z - r.o.y) * r.idir.y, b3 = (n1.w - r.o.y) * r.idir.y;
             const float b4 = (n2.z - r.o.z) * r.idir.z, b5 = (n2.w - r.o.z) * r.idir.z;
             const float c0min = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
-            const float c0max = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h.t));
+            const float c0max = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), s.h.t));
             const float c1min = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
-            const float c1max = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h.t));
+            const float c1max = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), s.h.t));
             const bool hit0 = c0min <= c0max * kBoxPad;
             const bool hit1 = c1min <= c1max * kBoxPad;
             uint32_t ca = __float_as_uint(n3.x), cb = __float_as_uint(n3.y);
             if (hit0 && hit1) {
                 if (c1min < c0min) { const uint32_t t = ca; ca = cb; cb = t; }
-                st.push(cb);
-                node = ca;
+                st.push(s.sp, cb);
+                s.node = ca;
             } else if (hit0) {
-                node = ca;
+                s.node = ca;
             } else if (hit1) {
-                node = cb;
+                s.node = cb;
+            } else if (s.sp == 0) {
+                going = false;
             } else {
-                if (st.sp == 0) return;
-                node = st.pop();
+                s.node = st.pop(s.sp);
             }
         }
-        const uint32_t first = node & kLeafFirstMask;
-        const uint32_t cnt = ((node >> kLeafCountShift) & 15u) + 1u;
-        if (COUNT) *nt += cnt;
-        for (uint32_t k = first; k < first + cnt; k++) {
-            const float4* tp = sc.tris + 3 * (size_t)k;
-            const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-            float tt, uu, vv;
-            if (watertight(r, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv)) {
-                const uint32_t prim = __float_as_uint(t0.w);
-                if (tt < h.t || (tt == h.t && prim < h.prim)) {
-                    h.t = tt;
-                    h.u = uu;
-                    h.v = vv;
-                    h.tri = k;
-                    h.prim = prim;
+        // leaf phase
+        if (going && (s.node & kLeafFlag)) {
+            const uint32_t first = s.node & kLeafFirstMask;
+            const uint32_t cnt = ((s.node >> kLeafCountShift) & 15u) + 1u;
+            if (COUNT) *nt += cnt;
+            for (uint32_t k = first; k < first + cnt; k++) {
+                if (COUNT && first_active_lane()) wi[1]++;
+                const float4* tp = sc.tris + 3 * (size_t)k;
+                const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                float tt, uu, vv;
+                if (watertight(s.r, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv)) {
+                    const uint32_t prim = __float_as_uint(t0.w);
+                    if (tt < s.h.t || (tt == s.h.t && prim < s.h.prim)) {
+                        s.h.t = tt;
+                        s.h.u = uu;
+                        s.h.v = vv;
+                        s.h.tri = k;
+                        s.h.prim = prim;
+                    }
                 }
             }
+            if (s.sp == 0) going = false;
+            else s.node = st.pop(s.sp);
         }
-        if (st.sp == 0) return;
-        node = st.pop();
+        const uint64_t g = __ballot(going);
+        if (g == 0) return;
+        if (batch < 64 && __popcll(__ballot(busy && !going)) >= batch) return;
     }
+}
+
+// closest hit, run to completion (query kernels and the picker)
+template <int BLOCK, bool COUNT = false>
+__device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, float tmax, HitFull& h,
+                                         LaneStack<BLOCK>& st, uint32_t* nn = nullptr, uint32_t* nt = nullptr) {
+    Trav s;
+    s.r = r;
+    s.h.t = tmax;
+    s.h.u = s.h.v = 0.f;
+    s.h.tri = kNoHit;
+    s.h.prim = kNoHit;
+    s.node = sc.root;
+    s.sp = 0;
+    bool going = true;
+    trav_run<BLOCK, COUNT>(sc, s, going, true, 64, 32, st, nn, nt);
+    h = s.h;
 }
 
 }  // namespace rtx
