@@ -105,6 +105,17 @@ struct LaneStack {
         else v = glob[(size_t)(sp - kStackLDS) * gstride];
         return v;
     }
+    // writes slot sp (the free slot above the top) without pushing
+    __device__ __forceinline__ void put(int sp, uint32_t v) {
+        if (sp < kStackLDS) lds[sp * BLOCK] = v;
+        else glob[(size_t)(sp - kStackLDS) * gstride] = v;
+    }
+    // reads the top entry (slot sp-1; slot 0 when empty, value unused then)
+    __device__ __forceinline__ uint32_t peek(int sp) {
+        const int i = sp > 0 ? sp - 1 : 0;
+        if (i < kStackLDS) return lds[i * BLOCK];
+        return glob[(size_t)(i - kStackLDS) * gstride];
+    }
 };
 
 struct HitFull {
@@ -182,20 +193,23 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             const float c1max = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), s.h.t));
             const bool hit0 = c0min <= c0max * kBoxPad;
             const bool hit1 = c1min <= c1max * kBoxPad;
-            uint32_t ca = __float_as_uint(n3.x), cb = __float_as_uint(n3.y);
-            if (hit0 && hit1) {
-                if (c1min < c0min) { const uint32_t t = ca; ca = cb; cb = t; }
-                st.push(s.sp, cb);
-                s.node = ca;
-            } else if (hit0) {
-                s.node = ca;
-            } else if (hit1) {
-                s.node = cb;
-            } else if (s.sp == 0) {
-                going = false;
-            } else {
-                s.node = st.pop(s.sp);
+            // branch-free child order and stack update: the far child is
+            // written to the free slot every step and kept only if both
+            // children are hit; the top slot is read every step and used
+            // only if neither is
+            const bool both = hit0 && hit1;
+            const bool take1 = both ? (c1min < c0min) : hit1;
+            const uint32_t ca = __float_as_uint(n3.x), cb = __float_as_uint(n3.y);
+            const uint32_t nearc = take1 ? cb : ca, farc = take1 ? ca : cb;
+            st.put(s.sp, farc);
+            const uint32_t top = st.peek(s.sp);
+            const bool any = hit0 || hit1;
+            s.sp += both ? 1 : 0;
+            if (!any) {
+                if (s.sp == 0) going = false;
+                s.sp = s.sp > 0 ? s.sp - 1 : 0;
             }
+            s.node = any ? nearc : top;
         }
         // leaf phase
         if (going && (s.node & kLeafFlag)) {
