@@ -64,6 +64,78 @@ def cpu_baseline(level, w, h, spp, depth, seconds, threads):
                       % (rows, w, h, spp, depth, level, dt, rays)}
 
 
+TUNED_JSON = os.path.join(ROOT, "rtxplay_amd", "tuned.json")
+ARCH = "gfx950"
+
+
+def batched_leg(reps, retune, log_tune):
+    """The 64-frame batch: 64 scenes (seeds 0..63) and cameras of uneven size
+    rendered by one hipGraph-captured persistent launch with autotuned launch
+    parameters, against the same 64 frames as 64 plain launches back to back
+    on one stream (the plain path's default parameters).  Returns the record."""
+    import rtxplay_amd as rtx
+    from rtxplay_amd import autotune
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import BatchFrames, batch_frame
+
+    t0 = time.perf_counter()
+    bf = BatchFrames([batch_frame(i) for i in range(64)])
+    log("batch: 64 scenes, %d triangles, %d samples, built in %.1f s"
+        % (bf.triangles(), bf.samples(), time.perf_counter() - t0))
+    stream = Stream()
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def sequential():
+        for l in bf.launchers:
+            l.ignite(stream=stream, sync=False)
+
+    seq_ms = timed(sequential)
+    rays = bf.rays()
+    log("batch: 64 plain launches %.1f ms (%d rays)" % (seq_ms, rays))
+
+    ctx = bf.scenes[0]
+    default = {k: ctx.tuning()[k] for k in autotune.KEYS}
+    b = rtx.Batch(ctx, bf.lps, use_graph=True)
+    def_ms = timed(lambda: b.launch(stream, sync=False))
+    b.free()
+    log("batch: one launch, default tuning %.1f ms" % def_ms)
+
+    tuned = None if retune else autotune.load(TUNED_JSON, ARCH, "batch64")
+    source = "stored"
+    if tuned is None:
+        # proxy: the same frames at a quarter of the spp (at least 1)
+        proxy = BatchFrames([batch_frame(i) for i in range(64)], spp_scale=0.25)
+        tuned, tsec, hist = autotune.tune_batch(proxy.scenes[0], proxy.lps, log=log_tune)
+        proxy.close()
+        autotune.save(TUNED_JSON, ARCH, "batch64", tuned, tsec,
+                      note="coordinate descent over %d configurations on the 64 frames at 1/4 spp" % len(hist))
+        source = "autotuned now (%d configurations)" % len(hist)
+    ctx.tuning(**tuned)
+    b = rtx.Batch(ctx, bf.lps, use_graph=True)
+    bat_ms = timed(lambda: b.launch(stream, sync=False))
+    b.free()
+    log("batch: one launch, tuned %s: %.1f ms" % (tuned, bat_ms))
+    # the batch reproduces the plain launches: rays per pixel summed over all frames
+    rays_b = bf.rays()
+    rec = {"frames": 64, "triangles": bf.triangles(), "samples": bf.samples(), "rays": rays,
+           "sequential_ms": round(seq_ms, 2), "batched_default_ms": round(def_ms, 2), "batched_ms": round(bat_ms, 2),
+           "speedup": round(seq_ms / bat_ms, 3), "Mrays_per_s_batched": round(rays / (bat_ms * 1e-3) / 1e6, 1),
+           "Mrays_per_s_sequential": round(rays / (seq_ms * 1e-3) / 1e6, 1), "tuning": tuned,
+           "tuning_source": source, "same_rays": rays_b == rays}
+    bf.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +146,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--retune", action="store_true", help="re-run the batch autotuner (else use the stored tuning)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only the post-process kernel over 64M pixels (768 MiB read, 256 MiB written): "
                          "a plain streaming kernel of known size that calibrates the HBM byte counters")
@@ -151,6 +226,14 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
 
+    launcher.free()
+    scene.close()
+
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(args.batch_reps, args.retune, log)
+        log("batched speedup %.2fx" % batched["speedup"])
+
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -174,11 +257,10 @@ def main():
                      "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 2),
                      "tri_tests_per_ray": round(n_tri, 2)},
         "cpu_baseline": cpu,
-        "batched_speedup": None, "bwd_fwd_ratio": None,
+        "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
+        "bwd_fwd_ratio": None,
     }
     print(json.dumps(out), flush=True)
-    launcher.free()
-    scene.close()
 
 
 if __name__ == "__main__":

@@ -158,12 +158,21 @@ rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, ui
  * traversal round's node phase ends when fewer than inner_frac/64 of the
  * traversing lanes are still on inner nodes; the lanes on a leaf then test
  * it (1..64; 1 = every lane reaches a leaf first, 64 = the first lane that
- * reaches a leaf ends the phase; default 36).  0 fields keep the current
- * value. */
+ * reaches a leaf ends the phase; default 36).  tile_w: pixels are handed
+ * out in tiles of tile_w x (64 / tile_w) (1, 2, 4, ..., 64; default 8).
+ * stack_lds: traversal stack entries per lane kept in LDS, the rest in a
+ * global overflow area (16 or 32; default 32).  blocks_per_cu: persistent
+ * grid of blocks_per_cu x CUs blocks of 256 lanes (1..max_blocks_per_cu, the
+ * occupancy limit, which is also the default; max_blocks_per_cu is read
+ * only).  0 fields keep the current value. */
 typedef struct {
     uint32_t shade_batch;
     uint32_t inner_frac;
-    uint32_t reserved[6];
+    uint32_t tile_w;
+    uint32_t stack_lds;
+    uint32_t blocks_per_cu;
+    uint32_t max_blocks_per_cu;
+    uint32_t reserved[2];
 } rtx_tuning;
 rtx_status rtx_tuning_get(rtx_ctx* ctx, rtx_tuning* t);
 rtx_status rtx_tuning_set(rtx_ctx* ctx, const rtx_tuning* t);
@@ -177,6 +186,22 @@ rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
  * Asynchronous on `stream` (the C++ Launcher syncs, as the reference does).
  * once != 0 launches the 1x1 picker path (lp->picker). */
 rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
+
+/* Batched render (no reference counterpart; SURVEY config 4): n frames, each
+ * with its own launch parameters (scene handle, camera, size, spp, depth,
+ * output buffers), rendered by ONE persistent kernel launch that pulls
+ * 64-pixel tiles of all frames from one work counter, largest frames first.
+ * Every frame's pixels are identical to rtx_launch of the same parameters.
+ * rtx_batch_create validates the frames and uploads the job table (the
+ * context's tuning at that moment applies); with use_graph != 0 the first
+ * rtx_batch_launch on a stream captures the launch into a hipGraph, which
+ * later launches replay (a graph needs a non-NULL stream; with NULL the
+ * batch launches directly).  Launches of one batch, and launches of its
+ * context, must not overlap. */
+typedef struct rtx_batch rtx_batch;
+rtx_status rtx_batch_create(rtx_ctx* ctx, const rtx_lp_general* lps, uint32_t n, int use_graph, rtx_batch** batch);
+rtx_status rtx_batch_launch(rtx_batch* batch, void* stream);
+rtx_status rtx_batch_destroy(rtx_batch* batch);
 
 /* Statistics launch: the same render as rtx_launch, which also writes per
  * pixel the number of BVH node visits and triangle tests of all its rays

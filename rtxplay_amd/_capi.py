@@ -90,7 +90,8 @@ class SceneStatsC(Structure):
 
 
 class TuningC(Structure):
-    _fields_ = [("shade_batch", c_uint32), ("inner_frac", c_uint32), ("reserved", c_uint32 * 6)]
+    _fields_ = [("shade_batch", c_uint32), ("inner_frac", c_uint32), ("tile_w", c_uint32), ("stack_lds", c_uint32),
+                ("blocks_per_cu", c_uint32), ("max_blocks_per_cu", c_uint32), ("reserved", c_uint32 * 2)]
 
 
 # every exported symbol: name -> (restype, argtypes)
@@ -115,6 +116,9 @@ SIGNATURES = {
     "rtx_tuning_get": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_tuning_set": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_launch": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),
+    "rtx_batch_create": (c_int, [c_void_p, c_void_p, c_uint32, c_int, POINTER(c_void_p)]),
+    "rtx_batch_launch": (c_int, [c_void_p, c_void_p]),
+    "rtx_batch_destroy": (c_int, [c_void_p]),
     "rtx_launch_counted": (c_int, [c_void_p, POINTER(LpGeneralC), c_void_p, c_void_p, c_void_p, c_void_p]),
     "rtx_trace": (c_int, [c_void_p, c_uint64, c_void_p, c_uint32, c_void_p, c_void_p]),
     "rtx_rng_draws": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p]),

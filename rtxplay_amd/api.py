@@ -316,7 +316,8 @@ class Scene:
             for k, v in kw.items():
                 setattr(t, k, int(v))
             call("rtx_tuning_set", self.ctx, byref(t))
-        return {"shade_batch": t.shade_batch, "inner_frac": t.inner_frac}
+        return {k: getattr(t, k) for k in ("shade_batch", "inner_frac", "tile_w", "stack_lds", "blocks_per_cu",
+                                           "max_blocks_per_cu")}
 
     def set_miss_color(self, ambient):
         call("rtx_set_miss_color", self.ctx, _f3(ambient))
@@ -409,6 +410,40 @@ class Launcher:
         return int(self._bufs["pick_id"].download(np.uint32, (1,))[0])
 
 
+class Batch:
+    """Many frames (each its own LpGeneral: scene handle, camera, size, spp,
+    depth, output buffers) rendered by one persistent launch; with
+    use_graph the first launch on a stream is captured into a hipGraph and
+    replayed after that.  Every frame equals Launcher.ignite of its LpGeneral.
+    The frames' buffers stay owned by the caller (e.g. one Launcher each)."""
+
+    def __init__(self, scene, lps, use_graph=True):
+        self.scene = scene
+        arr = (LpGeneralC * len(lps))()
+        for i, lp in enumerate(lps):
+            ctypes.memmove(ctypes.byref(arr, i * ctypes.sizeof(LpGeneralC)), ctypes.byref(lp),
+                           ctypes.sizeof(LpGeneralC))
+        p = c_void_p()
+        call("rtx_batch_create", scene.ctx, arr, len(lps), 1 if use_graph else 0, byref(p))
+        self.ptr = p
+
+    def launch(self, stream=None, sync=True):
+        call("rtx_batch_launch", self.ptr, _stream_ptr(stream))
+        if sync:
+            synchronize()
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            call("rtx_batch_destroy", self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def pp_sRGB(src, dst, w, h, stream=None):
     """optx/postproc.cu:56-61 (device pointers)"""
     call("rtx_pp_srgb", c_void_p(src), c_void_p(dst), int(w), int(h), _stream_ptr(stream))
@@ -439,6 +474,6 @@ def device_count():
     return n.value
 
 
-__all__ = ["Camera", "Thing", "Sphere", "Object", "Scene", "LpGeneral", "Launcher", "DeviceBuffer", "pp_sRGB",
+__all__ = ["Camera", "Thing", "Sphere", "Object", "Scene", "LpGeneral", "Launcher", "Batch", "DeviceBuffer", "pp_sRGB",
            "pp_none", "rng_draws", "device_count", "synchronize", "TYPE_DIFFUSE", "TYPE_REFLECT", "TYPE_REFRACT",
            "RtxError", "lib", "HitC", "RayC", "OpticsC"]

@@ -8,6 +8,7 @@
 // std::runtime_error from CUDA_CHECK/OPTX_CHECK, optx/util_cpu.h:18-53).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -110,10 +111,19 @@ const char* rtx_last_error(void) { return g_err.c_str(); }
 
 static rtx_status alloc_scratch(rtx::LaunchScratch& ls, uint32_t blocks_per_cu, int num_cus, int block) {
     ls.grid = blocks_per_cu * (uint32_t)num_cus;
+    ls.grid_max = ls.grid;
     ls.gstride = ls.grid * (uint32_t)block;
     HIP_OK(hipMalloc((void**)&ls.counter, sizeof(uint32_t)));
+    return RTX_SUCCESS;
+}
+
+// the traversal stack overflow area (tens of MB) is allocated by the first
+// launch that needs it, so contexts that only hold a scene for a batch cost
+// nothing beyond the scene
+static rtx_status ensure_gstack(rtx::LaunchScratch& ls) {
+    if (ls.gstack) return RTX_SUCCESS;
     HIP_OK(hipMalloc((void**)&ls.gstack,
-                     sizeof(uint32_t) * (size_t)(rtx::kMaxTreeDepth - rtx::kStackLDS) * ls.gstride));
+                     sizeof(uint32_t) * (size_t)(rtx::kMaxTreeDepth - rtx::kMinStackLDS) * ls.gstride));
     return RTX_SUCCESS;
 }
 
@@ -339,6 +349,10 @@ rtx_status rtx_tuning_get(rtx_ctx* c, rtx_tuning* t) {
     std::memset(t, 0, sizeof(*t));
     t->shade_batch = c->render_ls.shade_batch;
     t->inner_frac = c->render_ls.inner_frac;
+    t->tile_w = c->render_ls.tile_w;
+    t->stack_lds = c->render_ls.stack_lds;
+    t->blocks_per_cu = c->render_ls.grid / (uint32_t)c->num_cus;
+    t->max_blocks_per_cu = c->render_ls.grid_max / (uint32_t)c->num_cus;
     return RTX_SUCCESS;
 }
 
@@ -346,6 +360,16 @@ rtx_status rtx_tuning_set(rtx_ctx* c, const rtx_tuning* t) {
     if (!c || !t) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: NULL argument");
     if (t->shade_batch > 64) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: shade_batch > 64");
     if (t->inner_frac > 64) return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: inner_frac > 64");
+    if (t->tile_w > 64 || (t->tile_w & (t->tile_w - 1)))
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: tile_w must be a power of two <= 64");
+    if (t->stack_lds && t->stack_lds != 16 && t->stack_lds != 32)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: stack_lds must be 16 or 32");
+    if (t->blocks_per_cu * (uint32_t)c->num_cus > c->render_ls.grid_max)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_tuning_set: blocks_per_cu above the occupancy limit " +
+                                                 std::to_string(c->render_ls.grid_max / (uint32_t)c->num_cus));
+    if (t->tile_w) c->render_ls.tile_w = t->tile_w;
+    if (t->stack_lds) c->render_ls.stack_lds = t->stack_lds;
+    if (t->blocks_per_cu) c->render_ls.grid = t->blocks_per_cu * (uint32_t)c->num_cus;
     if (t->shade_batch) c->render_ls.shade_batch = t->shade_batch;
     if (t->inner_frac) c->render_ls.inner_frac = t->inner_frac;
     return RTX_SUCCESS;
@@ -528,6 +552,7 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
     if ((uint64_t)lp->image_w * lp->image_h > 0xFFFFFFFFull / 2)
         return fail(RTX_ERROR_LIMIT, "rtx_launch: image too large");
     HIP_OK(hipSetDevice(c->device));
+    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
     const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
     hipError_t e;
     if (once || lp->picker) {
@@ -545,6 +570,107 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
     return RTX_SUCCESS;
 }
 
+// ---- batched render -----------------------------------------------------
+}  // extern "C"
+
+struct rtx_batch {
+    rtx_ctx* ctx = nullptr;
+    rtx::Job* d_jobs = nullptr;
+    rtx::Job first{};
+    uint32_t njobs = 0, nwork = 0, tile_w = 8;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+rtx_status rtx_batch_create(rtx_ctx* c, const rtx_lp_general* lps, uint32_t n, int use_graph, rtx_batch** out) {
+    if (!c || !out || (n && !lps)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_create: NULL argument");
+    if (n == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_create: empty batch");
+    std::vector<uint32_t> order(n);
+    std::vector<Built*> built(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const rtx_lp_general& lp = lps[i];
+        built[i] = lookup(lp.is_handle);
+        if (!built[i]) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_batch_create: unknown is_handle in frame " + std::to_string(i));
+        if (built[i]->device != c->device) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_batch_create: handle on another device");
+        if (lp.image_w == 0 || lp.image_h == 0 || lp.spp == 0 || lp.depth == 0 || !lp.rawRGB)
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_create: bad launch parameters in frame " + std::to_string(i));
+        if ((uint64_t)lp.image_w * lp.image_h > 0xFFFFFFFFull / 2)
+            return fail(RTX_ERROR_LIMIT, "rtx_batch_create: image too large in frame " + std::to_string(i));
+        order[i] = i;
+    }
+    // longest work items first: a work item is one pixel whose cost grows
+    // with spp x depth, so frames are ordered by that, then by size
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const uint64_t ca = (uint64_t)lps[a].spp * lps[a].depth, cb = (uint64_t)lps[b].spp * lps[b].depth;
+        if (ca != cb) return ca > cb;
+        return (uint64_t)lps[a].image_w * lps[a].image_h > (uint64_t)lps[b].image_w * lps[b].image_h;
+    });
+    const uint32_t tw = c->render_ls.tile_w ? c->render_ls.tile_w : 8u;
+    const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
+    std::vector<rtx::Job> jobs(n);
+    uint64_t work = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t i = order[k];
+        jobs[k] = rtx::make_job(built[i]->hdr, lps[i], amb, tw, (uint32_t)work);
+        work = jobs[k].work_end;
+        if (work >= 0xFFFFFFFFull) return fail(RTX_ERROR_LIMIT, "rtx_batch_create: more than 2^32 pixel slots");
+    }
+    HIP_OK(hipSetDevice(c->device));
+    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    rtx_batch* b = new rtx_batch();
+    b->ctx = c;
+    b->njobs = n;
+    b->nwork = (uint32_t)work;
+    b->tile_w = tw;
+    b->first = jobs[0];
+    b->use_graph = use_graph != 0;
+    hipError_t e = hipMalloc((void**)&b->d_jobs, sizeof(rtx::Job) * n);
+    if (e == hipSuccess) e = hipMemcpy(b->d_jobs, jobs.data(), sizeof(rtx::Job) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b->d_jobs) (void)hipFree(b->d_jobs);
+        delete b;
+        return hip_fail(e, "rtx_batch_create: job table");
+    }
+    *out = b;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_batch_launch(rtx_batch* b, void* stream) {
+    if (!b) return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_launch: batch is NULL");
+    HIP_OK(hipSetDevice(b->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->use_graph || !s) {
+        hipError_t e = rtx::launch_render_batch(b->d_jobs, b->first, b->njobs, b->nwork, b->ctx->render_ls, b->tile_w, s);
+        if (e != hipSuccess) return hip_fail(e, "rtx_batch_launch");
+        return RTX_SUCCESS;
+    }
+    if (!b->exec) {
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = rtx::launch_render_batch(b->d_jobs, b->first, b->njobs, b->nwork, b->ctx->render_ls, b->tile_w, s);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "rtx_batch_launch: capture"); }
+        if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+        b->graph = g;
+        HIP_OK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_OK(hipGraphLaunch(b->exec, s));
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_batch_destroy(rtx_batch* b) {
+    if (!b) return RTX_SUCCESS;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    delete b;
+    return RTX_SUCCESS;
+}
+
 rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* node_visits, uint64_t* tri_tests,
                               uint64_t* wave_cycles, void* stream) {
     if (!c || !lp || !node_visits || !tri_tests) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: NULL argument");
@@ -553,6 +679,7 @@ rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* no
     if (lp->image_w == 0 || lp->image_h == 0 || lp->spp == 0 || lp->depth == 0 || !lp->rawRGB)
         return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: bad launch parameters");
     HIP_OK(hipSetDevice(c->device));
+    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
     rtx::TraceCounters tc;
     tc.nodes = node_visits;
     tc.tris = tri_tests;
@@ -569,6 +696,7 @@ rtx_status rtx_trace(rtx_ctx* c, uint64_t is_handle, const rtx_ray* rays, uint32
     if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_trace: unknown handle");
     if (n && (!rays || !hits)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_trace: NULL buffer");
     HIP_OK(hipSetDevice(c->device));
+    if (rtx_status st = ensure_gstack(c->trace_ls)) return st;
     hipError_t e = rtx::launch_trace(b->hdr, rays, n, hits, c->trace_ls, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "rtx_trace");
     return RTX_SUCCESS;

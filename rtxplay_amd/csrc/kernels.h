@@ -20,7 +20,22 @@ struct LaunchScratch {
     uint32_t grid = 0;             // persistent grid size in blocks
     uint32_t shade_batch = 16;     // render: finished lanes that trigger a wave's shading step
     uint32_t inner_frac = 36;      // traversal: node phase ends below inner_frac/64 of the going lanes
+    uint32_t tile_w = 8;           // render: work tile of tile_w x (64 / tile_w) pixels
+    uint32_t stack_lds = 32;       // render: traversal stack entries per lane in LDS (16 or 32)
+    uint32_t grid_max = 0;         // blocks the overflow stack area was sized for
 };
+
+// One frame of a launch: scene, launch parameters and its slice of the
+// global work index space (64 slots per tile).
+struct Job {
+    DevScene sc;
+    rtx_lp_general lp;
+    float3 ambient;
+    uint32_t tiles_x, tiles_y;
+    uint32_t work_begin, work_end;
+};
+
+Job make_job(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, uint32_t tile_w, uint32_t work_begin);
 
 // Optional per-pixel traversal statistics (node visits, triangle tests).
 struct TraceCounters {
@@ -33,6 +48,9 @@ hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 am
                          hipStream_t s);
 hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, float3 ambient,
                                  const LaunchScratch& ls, const TraceCounters& tc, hipStream_t s);
+// all jobs of a batch in one persistent launch (d_jobs sorted by work_begin)
+hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njobs, uint32_t nwork,
+                               const LaunchScratch& ls, uint32_t tile_w, hipStream_t s);
 hipError_t launch_pick(const DevScene& sc, const rtx_lp_general& lp, const LaunchScratch& ls, hipStream_t s);
 hipError_t launch_trace(const DevScene& sc, const rtx_ray* rays, uint32_t n, rtx_hit* hits,
                         const LaunchScratch& ls, hipStream_t s);

@@ -1,21 +1,28 @@
 // render.hip -- one launch renders a frame: raygen + closest hit + scatter +
 // spp accumulation, the work of __raygen__camera (optx/camera_r.cu:23-121),
 // __miss__ambient (optx/camera_r.cu:123-147) and the three closest-hit
-// programs (optx/optics_r.cu:22-431) under one optixLaunch.
+// programs (optx/optics_r.cu:22-431) under one optixLaunch.  The same kernel
+// renders a batch of frames (many scenes and cameras) in one launch.
 //
 // Shape: a persistent kernel.  Each lane owns one pixel at a time and walks
 // its samples one path after the other, so the pixel's Frand48 stream
 // (seeded with the pixel index) is consumed in exactly the reference order.
-// When a path ends the lane starts the pixel's next sample in the same loop
-// iteration; when the pixel is done it takes the next pixel from a global
-// counter (one atomic per wave, lanes ranked by ballot).  Pixels are handed
-// out in 8x8 tiles so that the primary rays of a wave are coherent.  Every
-// wave loops until the counter passes the last tile, then exits.
+// Work items are 64-pixel tiles (tile_w x 64/tile_w) of a job (one frame);
+// lanes take them from one global counter (one atomic per wave, lanes ranked
+// by ballot).  A batch lays its jobs out largest first, so the big frames
+// start early and the small ones fill the tail.  Every wave loops until the
+// counter passes the last work item, then exits.
+//
+// A wave traverses until a batch of its lanes hold a final closest hit,
+// shades those lanes (scatter, next ray, next sample or next pixel) and
+// resumes the others where they stopped (trav_run in traverse.h).
 //
 // The recursion of the reference becomes a loop with a throughput product;
 // the reference multiplies albedos on the way back up the recursion, this
 // multiplies them on the way down (same factors, other association).
 #include <hip/hip_runtime.h>
+
+#include <cstring>
 
 #include "kernels.h"
 #include "traverse.h"
@@ -25,15 +32,16 @@ namespace rtx {
 namespace {
 
 struct RenderArgs {
-    DevScene sc;
-    rtx_lp_general lp;
-    float3 ambient;
+    Job job;              // the frame of a single launch
+    const Job* jobs;      // batch: job table (device), sorted by work_begin
+    uint32_t njobs;
+    uint32_t nwork;       // total work items x 64
     uint32_t* counter;
     uint32_t* gstack;
     uint32_t gstride;
-    uint32_t tiles_x, tiles_y;
     uint32_t batch;       // lanes that must finish a ray before a wave shades (64: whole wave)
     uint32_t inner_frac;  // node phase ends below inner_frac/64 of the going lanes
+    uint32_t tile_log2w;  // tile width = 1 << tile_log2w, tile height = 64 / width
     TraceCounters tc;
 };
 
@@ -47,22 +55,30 @@ __device__ __forceinline__ void st(rtx_float3* p, float3 v) {
 
 __global__ void k_zero(uint32_t* p) { *p = 0u; }
 
-template <int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_render(const RenderArgs a) {
-    __shared__ uint32_t s_stack[kStackLDS * BLOCK];
-    LaneStack<BLOCK> stk;
+// job of a global work index: the last job whose work_begin <= wi
+__device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, uint32_t wi) {
+    uint32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].work_begin <= wi) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int BLOCK, bool COUNT, bool BATCH, int SLDS>
+__device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack) {
+    LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = a.gstack + blockIdx.x * BLOCK + threadIdx.x;
     stk.gstride = a.gstride;
 
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t W = a.lp.image_w, H = a.lp.image_h;
-    const uint32_t nwork = a.tiles_x * a.tiles_y * 64u;
-    const uint32_t spp = a.lp.spp, maxd = a.lp.depth;
-    const float inv_spp = 1.f / (float)spp;   // optx/camera_r.cu:112 with optx/v.h:44 (1/b * a)
-    const float3 eye = ld(a.lp.camera.eye), cu = ld(a.lp.camera.u), cv = ld(a.lp.camera.v);
-    const float3 hvec = ld(a.lp.camera.hvec), wvec = ld(a.lp.camera.wvec), dvec = ld(a.lp.camera.dvec);
-    const float half_ap = a.lp.camera.aperture / 2.f;
+    const uint32_t tw = 1u << a.tile_log2w;
+
+    // the lane's job: uniform for a single launch, per lane for a batch
+    const Job* J = BATCH ? a.jobs : &a.job;
+    DevScene sc = a.job.sc;
+    uint32_t W = a.job.lp.image_w, H = a.job.lp.image_h, spp = a.job.lp.spp, maxd = a.job.lp.depth;
 
     bool active = false, exhausted = false, going = false;
     uint32_t x = 0, y = 0, sample = 0, k = 0, rpp = 0;
@@ -72,18 +88,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     float3 thr = f3(1, 1, 1);
     float3 acc = f3(0, 0, 0), nacc = f3(0, 0, 0), aacc = f3(0, 0, 0), gN = f3(0, 0, 0), gA = f3(0, 0, 0);
     Trav tv;
-    trav_begin(tv, a.sc, f3(0, 0, 0), f3(0, 0, 1), 1e-3f, 1e16f);
+    trav_begin(tv, sc, f3(0, 0, 0), f3(0, 0, 1), 1e-3f, 1e16f);
     uint64_t cyc_trav = 0;
     uint32_t wave_iters[2] = {0, 0};
     const uint64_t cyc_start = COUNT ? clock64() : 0;
 
     // optx/camera_r.cu:69-99 and optx/camera.h:67-77: the sample's primary ray
     auto start_sample = [&]() {
+        const rtx_camera& cam = J->lp.camera;
         const float s = 2.f * ((float)x + rng.next()) / (float)W - 1.f;
         const float t = 2.f * ((float)y + rng.next()) / (float)H - 1.f;
-        const float3 rd = half_ap * rnd_in_disk(rng);
-        const float3 off = rd.x * cu + rd.y * cv;
-        trav_begin(tv, a.sc, eye + off, ((s * wvec + t * hvec) - dvec) - off, 1e-3f, 1e16f);
+        const float3 rd = (cam.aperture / 2.f) * rnd_in_disk(rng);
+        const float3 off = rd.x * ld(cam.u) + rd.y * ld(cam.v);
+        trav_begin(tv, sc, ld(cam.eye) + off, ((s * ld(cam.wvec) + t * ld(cam.hvec)) - ld(cam.dvec)) - off, 1e-3f,
+                   1e16f);
         going = true;
         k = 0;
         thr = f3(1.f, 1.f, 1.f);
@@ -92,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     };
 
     for (;;) {
-        // lanes without a pixel take one; out-of-image tile slots are skipped
+        // lanes without a pixel take one; tile slots outside the frame are skipped
         for (;;) {
             const bool need = !active && !exhausted;
             const uint64_t mask = __ballot(need);
@@ -103,12 +121,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
             base = __shfl(base, leader, 64);
             if (need) {
                 const uint32_t wi = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                if (wi >= nwork) {
+                if (wi >= a.nwork) {
                     exhausted = true;
                 } else {
-                    const uint32_t tile = wi >> 6;
-                    x = (tile % a.tiles_x) * 8u + (wi & 7u);
-                    y = (tile / a.tiles_x) * 8u + ((wi >> 3) & 7u);
+                    if (BATCH) {
+                        J = a.jobs + find_job(a.jobs, a.njobs, wi);
+                        sc = J->sc;
+                        W = J->lp.image_w;
+                        H = J->lp.image_h;
+                        spp = J->lp.spp;
+                        maxd = J->lp.depth;
+                    }
+                    const uint32_t li = wi - J->work_begin;
+                    const uint32_t tile = li >> 6, in = li & 63u;
+                    x = (tile % J->tiles_x) * tw + (in & (tw - 1u));
+                    y = (tile / J->tiles_x) * (64u / tw) + (in >> a.tile_log2w);
                     if (x < W && y < H) {
                         active = true;
                         rng.init(W * y + x);              // optx/camera_r.cu:35, :42-43
@@ -126,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT>(a.sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+        trav_run<BLOCK, COUNT>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
                                wave_iters);
         if (COUNT) cyc_trav += clock64() - c0;
         if (!active || going) continue;
@@ -139,16 +166,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
             // __miss__ambient, optx/camera_r.cu:135-146
             const float3 unit = unitv(d);
             const float t = .5f * (unit.y + 1.f);
-            col = thr * ((1.f - t) * f3(1.f, 1.f, 1.f) + t * a.ambient);
+            col = thr * ((1.f - t) * f3(1.f, 1.f, 1.f) + t * J->ambient);
             end = true;
         } else if (k == maxd) {
             end = true;                                    // optx/optics_r.cu:34-42
         } else {
             // hit reconstruction, optx/optics_r.cu:45-90
-            const float4* tp = a.sc.tris + 3 * (size_t)tv.h.tri;
+            const float4* tp = sc.tris + 3 * (size_t)tv.h.tri;
             const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
             const uint32_t inst = __float_as_uint(t1.w);
-            const float4 m0 = a.sc.insts[2 * inst], m1 = a.sc.insts[2 * inst + 1];
+            const float4 m0 = sc.insts[2 * inst], m1 = sc.insts[2 * inst + 1];
             const float3 A = xyz(t0), B = xyz(t1), C = xyz(t2);
             const float w = 1.f - tv.h.u - tv.h.v;
             const float3 hp = (w * A + tv.h.u * B) + tv.h.v * C;
@@ -194,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
                 }
             }
             if (!end) {
-                trav_begin(tv, a.sc, hp, nd, 1e-3f, 1e16f);   // optx/optics_r.cu:115-132
+                trav_begin(tv, sc, hp, nd, 1e-3f, 1e16f);   // optx/optics_r.cu:115-132
                 going = true;
             }
         }
@@ -205,12 +232,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
             nacc = nacc + gN;
             aacc = aacc + gA;
             if (++sample == spp) {
+                const rtx_lp_general& lp = J->lp;
                 const uint32_t pix = W * y + x;            // optx/camera_r.cu:111-120
+                const float inv_spp = 1.f / (float)spp;    // optx/v.h:44: 1/b * a
                 const float3 mean = inv_spp * acc;
-                st(&a.lp.rawRGB[pix], f3(clampf(mean.x, 0.f, 1.f), clampf(mean.y, 0.f, 1.f), clampf(mean.z, 0.f, 1.f)));
-                if (a.lp.rpp) a.lp.rpp[pix] = rpp;
-                if (a.lp.normals) st(&a.lp.normals[pix], inv_spp * nacc);
-                if (a.lp.albedos) st(&a.lp.albedos[pix], inv_spp * aacc);
+                st(&lp.rawRGB[pix], f3(clampf(mean.x, 0.f, 1.f), clampf(mean.y, 0.f, 1.f), clampf(mean.z, 0.f, 1.f)));
+                if (lp.rpp) lp.rpp[pix] = rpp;
+                if (lp.normals) st(&lp.normals[pix], inv_spp * nacc);
+                if (lp.albedos) st(&lp.albedos[pix], inv_spp * aacc);
                 if (COUNT) {
                     a.tc.nodes[pix] = cnt_nodes;
                     a.tc.tris[pix] = cnt_tris;
@@ -231,6 +260,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     }
 }
 
+// single frame: 96 VGPRs and (with 32 LDS stack entries) 32 KB of LDS per
+// block allow 5 waves per SIMD
+template <int BLOCK, bool COUNT, int SLDS>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_render(const RenderArgs a) {
+    __shared__ uint32_t s_stack[SLDS * BLOCK];
+    render_body<BLOCK, COUNT, false, SLDS>(a, s_stack);
+}
+
+// batch: per-lane job state needs more registers (4 waves per SIMD)
+template <int BLOCK, int SLDS>
+__global__ __launch_bounds__(BLOCK) void k_render_batch(const RenderArgs a) {
+    __shared__ uint32_t s_stack[SLDS * BLOCK];
+    render_body<BLOCK, false, true, SLDS>(a, s_stack);
+}
+
 // picker: one primary ray through (pick_x, pick_y), optx/camera_r.cu:26-32 and
 // optx/optics_r.cu:24-28 / optx/camera_r.cu:125-129
 __global__ __launch_bounds__(64) void k_pick(const RenderArgs a) {
@@ -240,43 +284,61 @@ __global__ __launch_bounds__(64) void k_pick(const RenderArgs a) {
     stk.lds = (lds_u32*)s_stack;
     stk.glob = a.gstack;
     stk.gstride = a.gstride;
-    const uint32_t W = a.lp.image_w, H = a.lp.image_h;
-    const uint32_t x = a.lp.pick_x, y = a.lp.pick_y;
+    const rtx_lp_general& lp = a.job.lp;
+    const uint32_t W = lp.image_w, H = lp.image_h;
+    const uint32_t x = lp.pick_x, y = lp.pick_y;
     Frand48 rng;
     rng.init(W * y + x);
     const float s = 2.f * ((float)x + rng.next()) / (float)W - 1.f;
     const float t = 2.f * ((float)y + rng.next()) / (float)H - 1.f;
-    const float3 rd = (a.lp.camera.aperture / 2.f) * rnd_in_disk(rng);
-    const float3 off = rd.x * ld(a.lp.camera.u) + rd.y * ld(a.lp.camera.v);
-    const float3 o = ld(a.lp.camera.eye) + off;
-    const float3 d = ((s * ld(a.lp.camera.wvec) + t * ld(a.lp.camera.hvec)) - ld(a.lp.camera.dvec)) - off;
+    const float3 rd = (lp.camera.aperture / 2.f) * rnd_in_disk(rng);
+    const float3 off = rd.x * ld(lp.camera.u) + rd.y * ld(lp.camera.v);
+    const float3 o = ld(lp.camera.eye) + off;
+    const float3 d = ((s * ld(lp.camera.wvec) + t * ld(lp.camera.hvec)) - ld(lp.camera.dvec)) - off;
     TRay r;
     tray_init(r, o, d, 1e-3f);
     HitFull h;
-    traverse<64>(a.sc, r, 1e16f, h, stk);
-    *a.lp.pick_id = h.tri == kNoHit ? 0xFFFFFFFFu : __float_as_uint(a.sc.tris[3 * (size_t)h.tri + 1].w);
+    traverse<64>(a.job.sc, r, 1e16f, h, stk);
+    *lp.pick_id = h.tri == kNoHit ? 0xFFFFFFFFu : __float_as_uint(a.job.sc.tris[3 * (size_t)h.tri + 1].w);
 }
 
-RenderArgs make_args(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls) {
+uint32_t log2u(uint32_t v) {
+    uint32_t l = 0;
+    while ((1u << (l + 1)) <= v) l++;
+    return l;
+}
+
+RenderArgs base_args(const LaunchScratch& ls) {
     RenderArgs a;
-    a.sc = sc;
-    a.lp = lp;
-    a.ambient = ambient;
+    std::memset(&a, 0, sizeof(a));
     a.counter = ls.counter;
     a.gstack = ls.gstack;
     a.gstride = ls.gstride;
-    a.tiles_x = (lp.image_w + 7u) / 8u;
-    a.tiles_y = (lp.image_h + 7u) / 8u;
     a.batch = ls.shade_batch ? ls.shade_batch : 16u;
     a.inner_frac = ls.inner_frac ? ls.inner_frac : 36u;
+    a.tile_log2w = log2u(ls.tile_w ? ls.tile_w : 8u);
     return a;
 }
 
 }  // namespace
 
+Job make_job(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, uint32_t tile_w, uint32_t work_begin) {
+    Job j;
+    std::memset(&j, 0, sizeof(j));
+    j.sc = sc;
+    j.lp = lp;
+    j.ambient = ambient;
+    const uint32_t th = 64u / tile_w;
+    j.tiles_x = (lp.image_w + tile_w - 1u) / tile_w;
+    j.tiles_y = (lp.image_h + th - 1u) / th;
+    j.work_begin = work_begin;
+    j.work_end = work_begin + j.tiles_x * j.tiles_y * 64u;
+    return j;
+}
+
 uint32_t render_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false>, kRenderBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false, kStackLDS>, kRenderBlock, 0) !=
             hipSuccess || nb < 1)
         nb = 1;
     return (uint32_t)nb;
@@ -284,24 +346,52 @@ uint32_t render_blocks_per_cu() {
 
 hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,
                          hipStream_t s) {
-    RenderArgs a = make_args(sc, lp, ambient, ls);
-    a.tc = TraceCounters();
+    RenderArgs a = base_args(ls);
+    a.job = make_job(sc, lp, ambient, 1u << a.tile_log2w, 0);
+    a.njobs = 1;
+    a.nwork = a.job.work_end;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-    hipLaunchKernelGGL((k_render<kRenderBlock, false>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    if (ls.stack_lds == 16)
+        hipLaunchKernelGGL((k_render<kRenderBlock, false, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_render<kRenderBlock, false, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, float3 ambient,
                                  const LaunchScratch& ls, const TraceCounters& tc, hipStream_t s) {
-    RenderArgs a = make_args(sc, lp, ambient, ls);
+    RenderArgs a = base_args(ls);
+    a.job = make_job(sc, lp, ambient, 1u << a.tile_log2w, 0);
+    a.njobs = 1;
+    a.nwork = a.job.work_end;
     a.tc = tc;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-    hipLaunchKernelGGL((k_render<kRenderBlock, true>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    if (ls.stack_lds == 16)
+        hipLaunchKernelGGL((k_render<kRenderBlock, true, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_render<kRenderBlock, true, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njobs, uint32_t nwork,
+                               const LaunchScratch& ls, uint32_t tile_w, hipStream_t s) {
+    RenderArgs a = base_args(ls);
+    a.tile_log2w = log2u(tile_w);
+    a.job = first;
+    a.jobs = d_jobs;
+    a.njobs = njobs;
+    a.nwork = nwork;
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
+    if (ls.stack_lds == 16)
+        hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_pick(const DevScene& sc, const rtx_lp_general& lp, const LaunchScratch& ls, hipStream_t s) {
-    RenderArgs a = make_args(sc, lp, make_float3(0.f, 0.f, 0.f), ls);
+    RenderArgs a = base_args(ls);
+    a.job = make_job(sc, lp, make_float3(0.f, 0.f, 0.f), 8u, 0);
     hipLaunchKernelGGL(k_pick, dim3(1), dim3(64), 0, s, a);
     return hipGetLastError();
 }

@@ -29,7 +29,8 @@ constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 constexpr uint32_t kMaxLeaf = 8;          // triangles per leaf (<= 16 fits the encoding)
 constexpr uint32_t kMaxPrims = kLeafFirstMask;  // 134M triangles per scene
 constexpr int kMaxTreeDepth = 64;         // guaranteed by the builder
-constexpr int kStackLDS = 32;             // traversal stack entries per lane kept in LDS
+constexpr int kStackLDS = 32;             // traversal stack entries per lane kept in LDS (default)
+constexpr int kMinStackLDS = 16;          // smallest LDS stack a render launch can be tuned to
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyRoot = 0xFFFFFFFFu;
 

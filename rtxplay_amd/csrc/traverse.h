@@ -88,33 +88,33 @@ __device__ __forceinline__ bool watertight(const TRay& r, float3 v0, float3 v1, 
 // ds_read/ds_write instead of flat loads through a generic pointer
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-template <int BLOCK>
+template <int BLOCK, int SLDS = kStackLDS>
 struct LaneStack {
     lds_u32* lds;      // LDS column of this lane: entry i at lds[i * BLOCK]
-    uint32_t* glob;    // overflow column: entry i at glob[(i - kStackLDS) * gstride]
+    uint32_t* glob;    // overflow column: entry i at glob[(i - SLDS) * gstride]
     uint32_t gstride;
     __device__ __forceinline__ void push(int& sp, uint32_t v) {
-        if (sp < kStackLDS) lds[sp * BLOCK] = v;
-        else glob[(size_t)(sp - kStackLDS) * gstride] = v;
+        if (sp < SLDS) lds[sp * BLOCK] = v;
+        else glob[(size_t)(sp - SLDS) * gstride] = v;
         sp++;
     }
     __device__ __forceinline__ uint32_t pop(int& sp) {
         sp--;
         uint32_t v;
-        if (sp < kStackLDS) v = lds[sp * BLOCK];
-        else v = glob[(size_t)(sp - kStackLDS) * gstride];
+        if (sp < SLDS) v = lds[sp * BLOCK];
+        else v = glob[(size_t)(sp - SLDS) * gstride];
         return v;
     }
     // writes slot sp (the free slot above the top) without pushing
     __device__ __forceinline__ void put(int sp, uint32_t v) {
-        if (sp < kStackLDS) lds[sp * BLOCK] = v;
-        else glob[(size_t)(sp - kStackLDS) * gstride] = v;
+        if (sp < SLDS) lds[sp * BLOCK] = v;
+        else glob[(size_t)(sp - SLDS) * gstride] = v;
     }
     // reads the top entry (slot sp-1; slot 0 when empty, value unused then)
     __device__ __forceinline__ uint32_t peek(int sp) {
         const int i = sp > 0 ? sp - 1 : 0;
-        if (i < kStackLDS) return lds[i * BLOCK];
-        return glob[(size_t)(i - kStackLDS) * gstride];
+        if (i < SLDS) return lds[i * BLOCK];
+        return glob[(size_t)(i - SLDS) * gstride];
     }
 };
 
@@ -161,9 +161,9 @@ __device__ __forceinline__ bool first_active_lane() {
 // least `batch` of the wave's `busy` lanes have finished, so that the caller
 // can shade them while the others keep their state and resume later.
 // COUNT adds node-visit and triangle-test counts (statistics launches only).
-template <int BLOCK, bool COUNT = false>
+template <int BLOCK, bool COUNT = false, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
-                                         int inner_frac, LaneStack<BLOCK>& st, uint32_t* nn = nullptr,
+                                         int inner_frac, Stack& st, uint32_t* nn = nullptr,
                                          uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
     if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
@@ -242,9 +242,9 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
 }
 
 // closest hit, run to completion (query kernels and the picker)
-template <int BLOCK, bool COUNT = false>
+template <int BLOCK, bool COUNT = false, class Stack>
 __device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, float tmax, HitFull& h,
-                                         LaneStack<BLOCK>& st, uint32_t* nn = nullptr, uint32_t* nt = nullptr) {
+                                         Stack& st, uint32_t* nn = nullptr, uint32_t* nt = nullptr) {
     Trav s;
     s.r = r;
     s.h.t = tmax;

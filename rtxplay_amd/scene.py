@@ -113,3 +113,76 @@ class RTOWScene(Scene):
 
 # camera of optx/rtwo.cxx:100-108
 RTOW_CAMERA = dict(eye=(13., 2., 3.), pat=(0., 0., 0.), vup=(0., 1., 0.), fov=20., aperture=.1, fostance=10.)
+
+
+# ---------------------------------------------------------------------------
+# the 64-frame batch workload: independent scenes and cameras of uneven size
+# ---------------------------------------------------------------------------
+BATCH_PARAM_SEED = 0x0BA7C400
+_BATCH_DRAWS = 6
+
+
+def batch_frame(i):
+    """Parameters of frame i of the batch workload.
+
+    The scene is sphere_things(seed=i, half) with every sphere at one
+    subdivision level.  The other parameters come from one Frand48 stream
+    (seed BATCH_PARAM_SEED) of which frame i takes draws 6i .. 6i+5 (one
+    stream, because drand48 streams of consecutive seeds move in lock step),
+    in this order: half-extent U{3..11}, level U{2..6}, resolution scale s
+    log-uniform in [1, 6] (w x h = 2*round(160 s) x 2*round(90 s), i.e.
+    320x180 .. 1920x1080), spp log-uniform in [4, 64], lookfrom angle
+    U[0, 2 pi) on a radius-13 ring, lookfrom height U[1, 4].  Camera: lookat
+    (0,0,0), vup y, vfov 20, aperture 0.1, focus distance 10; depth 16."""
+    r = Frand48(BATCH_PARAM_SEED)
+    for _ in range(_BATCH_DRAWS * int(i)):
+        r()
+    half = 3 + min(int(r() * 9), 8)
+    level = 2 + min(int(r() * 5), 4)
+    s = 6.0 ** float(r())
+    w, h = min(2 * int(round(160 * s)), 1920), min(2 * int(round(90 * s)), 1080)
+    spp = int(round(4.0 * 16.0 ** float(r())))
+    ang = 2.0 * np.pi * float(r())
+    height = 1.0 + 3.0 * float(r())
+    eye = (13.0 * np.cos(ang), height, 13.0 * np.sin(ang))
+    return dict(index=int(i), seed=int(i), half=half, level=level, w=w, h=h, spp=spp, depth=16,
+                camera=dict(eye=eye, pat=(0., 0., 0.), vup=(0., 1., 0.), fov=20., aperture=.1, fostance=10.))
+
+
+class BatchFrames:
+    """Builds the scenes of a list of batch frames (one Scene, i.e. one
+    context and acceleration structure, per frame), one Launcher per frame
+    for its output buffers, and the LpGeneral records."""
+
+    def __init__(self, frames, device=0, spp_scale=1.0):
+        from .api import Camera, Launcher, LpGeneral
+        self.frames = frames
+        self.scenes, self.launchers, self.lps = [], [], []
+        for f in frames:
+            sc = RTOWScene(seed=f["seed"], half=f["half"], level=f["level"], device=device)
+            sc.load()
+            handle = sc.build()
+            lp = LpGeneral()
+            lp.image_w, lp.image_h, lp.depth, lp.is_handle = f["w"], f["h"], f["depth"], handle
+            lp.spp = max(1, int(round(f["spp"] * spp_scale)))
+            c = f["camera"]
+            lp.camera = Camera().set(c["eye"], c["pat"], c["vup"], c["fov"], f["w"] / f["h"], c["aperture"],
+                                     c["fostance"]).c
+            self.scenes.append(sc)
+            self.launchers.append(Launcher(sc, lp))
+            self.lps.append(lp)
+
+    def triangles(self):
+        return sum(sc.stats()["num_prims"] for sc in self.scenes)
+
+    def samples(self):
+        return sum(lp.image_w * lp.image_h * lp.spp for lp in self.lps)
+
+    def rays(self):
+        return sum(int(l.rays_per_pixel().sum(dtype=np.uint64)) for l in self.launchers)
+
+    def close(self):
+        for l in self.launchers:
+            l.free()
+        for sc in self.scenes:
+            sc.close()
