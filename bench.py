@@ -136,6 +136,76 @@ def batched_leg(reps, retune, log_tune):
     return rec
 
 
+def diff_leg(reps):
+    """The differentiable frame: level-4 scene, 640x360, 64 spp, depth 8.
+    Forward = one render (rtx_launch); backward = one adjoint path-replay
+    launch (rtx_launch_adjoint) for an L2 image loss against a target
+    rendered with seeded materials.  Returns the timing record."""
+    import ctypes
+    import rtxplay_amd as rtx
+    from rtxplay_amd import TYPE_DIFFUSE, TYPE_REFLECT
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene
+
+    w, h, spp, depth = 640, 360, 64, 8
+    sc = RTOWScene(seed=0, half=11, level=4)
+    sc.load()
+    handle = sc.build()
+    lp = rtx.LpGeneral()
+    lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, depth, handle
+    lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0).c
+    la = rtx.Launcher(sc, lp)
+    types = [t["type"] for t in sc.things_spec]
+    m0 = sc.get_materials()
+    g = np.random.default_rng(7)
+    mt = m0.copy()
+    for k, t in enumerate(types):
+        if t in (TYPE_DIFFUSE, TYPE_REFLECT):
+            mt[k, :3] = np.clip(m0[k, :3] * g.uniform(.6, 1.3, 3), .02, 1.)
+        if t == TYPE_REFLECT:
+            mt[k, 3] = g.uniform(0, .5)
+    sc.set_materials(mt)
+    la.ignite()
+    T = la.raw_rgb()
+    sc.set_materials(m0)
+    la.ignite()
+    I = la.raw_rgb()
+    dldi = rtx.DeviceBuffer.from_array((2.0 * (I - T)).astype(np.float32))
+    dpar = rtx.DeviceBuffer(16 * len(types))
+    stream = Stream()
+
+    def fwd():
+        la.ignite(stream=stream, sync=False)
+
+    def bwd():
+        rtx.api.call("rtx_launch_adjoint", sc.ctx, ctypes.byref(lp), dldi.ptr, dpar.ptr, stream.ptr)
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    f_ms = timed(fwd)
+    b_ms = timed(bwd)
+    nparam = sum(3 for t in types if t == TYPE_DIFFUSE) + sum(4 for t in types if t == TYPE_REFLECT)
+    log("diff: forward %.2f ms, adjoint %.2f ms, ratio %.2f (%d material parameters)"
+        % (f_ms, b_ms, b_ms / f_ms, nparam))
+    rec = {"workload": "rtow-final-scene level-4 spheres, 640x360, 64 spp, depth 8, L2 image loss",
+           "forward_ms": round(f_ms, 3), "adjoint_ms": round(b_ms, 3), "ratio": round(b_ms / f_ms, 3),
+           "parameters": nparam}
+    dldi.free()
+    dpar.free()
+    la.free()
+    sc.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +217,7 @@ def main():
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--retune", action="store_true", help="re-run the batch autotuner (else use the stored tuning)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -234,6 +305,10 @@ def main():
         batched = batched_leg(args.batch_reps, args.retune, log)
         log("batched speedup %.2fx" % batched["speedup"])
 
+    diff = None
+    if not args.no_diff:
+        diff = diff_leg(max(3, args.batch_reps))
+
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -258,7 +333,7 @@ def main():
                      "tri_tests_per_ray": round(n_tri, 2)},
         "cpu_baseline": cpu,
         "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
-        "bwd_fwd_ratio": None,
+        "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
     }
     print(json.dumps(out), flush=True)
 

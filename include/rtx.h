@@ -203,6 +203,32 @@ rtx_status rtx_batch_create(rtx_ctx* ctx, const rtx_lp_general* lps, uint32_t n,
 rtx_status rtx_batch_launch(rtx_batch* batch, void* stream);
 rtx_status rtx_batch_destroy(rtx_batch* batch);
 
+/* ---- differentiable rendering (no reference counterpart; SURVEY config 5) --
+ * Material parameters are float4 per instance (instance = thing id):
+ * (albedo r, g, b, fuzz).  rtx_scene_set_materials writes them into the
+ * built scene on the device (diffuse instances take the albedo, metal
+ * instances albedo and fuzz, dielectric instances nothing); they hold until
+ * the next rtx_scene_build / rtx_scene_update.  rtx_scene_get_materials
+ * reads the built scene's values (for a dielectric: 0, 0, 0, ior).  params
+ * are device pointers to num_things x 4 floats. */
+rtx_status rtx_scene_set_materials(rtx_ctx* ctx, uint64_t is_handle, const float* params, void* stream);
+rtx_status rtx_scene_get_materials(rtx_ctx* ctx, uint64_t is_handle, float* params, void* stream);
+
+/* Adjoint of rtx_launch w.r.t. the material parameters (path replay): given
+ * the forward image in lp->rawRGB (the output of rtx_launch with the same
+ * lp) and dL_drgb = dL/d rawRGB (device float3 per pixel), writes
+ * d_params = dL/d params (device float4 per instance: d albedo rgb, d fuzz;
+ * overwritten).  The kernel replays every sample path with the forward's
+ * per-pixel random streams and propagates the loss gradient back along each
+ * path that reaches the sky: albedo by the product rule, fuzz through the
+ * chain of metal and dielectric directions to the sky colour.  Gradients
+ * hold the discrete path structure fixed (hit triangles, reflect/refract and
+ * absorb decisions), the interior derivative; pixels whose mean clamps to 0
+ * or 1 pass no gradient.  Sums use float atomics, so the last bits vary
+ * from run to run. */
+rtx_status rtx_launch_adjoint(rtx_ctx* ctx, const rtx_lp_general* lp, const rtx_float3* dL_drgb, float* d_params,
+                              void* stream);
+
 /* Statistics launch: the same render as rtx_launch, which also writes per
  * pixel the number of BVH node visits and triangle tests of all its rays
  * (device uint64 arrays of image_w*image_h) and, if wave_cycles is not NULL,

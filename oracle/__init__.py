@@ -68,6 +68,8 @@ def lib():
             "orc_decidable": (c_int, [c_double, c_double, c_double, c_double, c_double, c_double]),
             "orc_render": (c_uint64, [POINTER(SceneC), POINTER(CameraC), c_uint32, c_uint32, c_uint32, c_uint32,
                                       c_uint32, c_uint32, c_int, P, P, P, P, P, c_uint32]),
+            "orc_render_frozen": (None, [POINTER(SceneC), POINTER(CameraC), c_uint32, c_uint32, c_uint32, c_uint32,
+                                         c_uint32, c_uint32, c_int, P, P]),
             "orc_rtow_render": (c_uint64, [c_int, c_int, c_int, c_int, c_int, P]),
             "orc_rtow_scene": (c_uint32, [c_int, P, c_uint32]),
             "orc_pp_srgb": (None, [P, P, c_int, c_int]),
@@ -291,6 +293,26 @@ class Scene:
                                 _p(raw), _p(rpp), _p(nrm), _p(alb), _p(sq) if sq is not None else None,
                                 seed_base)
         return dict(raw=raw, rpp=rpp, normals=nrm, albedos=alb, sumsq=sq, rays=rays)
+
+
+    def materials(self):
+        """float32[n,4]: albedo rgb and fuzz of every thing"""
+        m = np.zeros((len(self.things), 4), np.float32)
+        for k in range(len(self.things)):
+            m[k, :3] = list(self.things[k].albedo)
+            m[k, 3] = self.things[k].fuzz
+        return m
+
+    def render_frozen(self, cam, w, h, spp, depth, params, y0=0, y1=None, nthreads=8):
+        """Paths traced with the scene's own materials, radiance evaluated in
+        FP64 with params (float32[n,4]) along the same vertices -> float64[h,w,3]."""
+        y1 = h if y1 is None else y1
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 4)
+        assert len(params) == len(self.things)
+        raw = np.zeros((h, w, 3), np.float64)
+        lib().orc_render_frozen(ctypes.byref(self.c), ctypes.byref(cam), w, h, spp, depth, y0, y1, nthreads,
+                                _p(params), _p(raw))
+        return raw
 
 
 def rtow_render(w=1280, h=720, spp=10, depth=50, argorder=1):

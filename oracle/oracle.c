@@ -893,6 +893,155 @@ uint64_t orc_render(const orc_scene* s, const orc_camera* cam, uint32_t w, uint3
 }
 
 /* ========================================================================= */
+/* Frozen-structure evaluation (finite-difference checker for material      */
+/* gradients).  A sample path is traced exactly as trace_rec does it (same   */
+/* random draws, same decisions) with the scene's own materials, its        */
+/* vertices are recorded, and its radiance is then evaluated in FP64 along   */
+/* the same vertices with other material parameters: directions follow the  */
+/* recorded normals, random vectors and choices; nothing is re-traced.       */
+/* Scatter formulas: optx/optics_r.cu:108 (diffuse), :235-242 (metal),       */
+/* :372-398 (dielectric); sky optx/camera_r.cu:135-140.                       */
+/* ========================================================================= */
+
+typedef struct { int type; uint32_t thing; f3 N; f3 rv; float ratio; int refl; } fvert;
+
+/* returns 1 if the path reaches the sky, 0 if it ends black */
+static int frozen_trace(const orc_scene* s, uint32_t maxd, orc_frand48* rng, f3 ori, f3 dir, fvert* vs, int* nv) {
+    uint32_t dep = 0;
+    *nv = 0;
+    for (;;) {
+        wray r;
+        float o[3] = {ori.x, ori.y, ori.z}, d[3] = {dir.x, dir.y, dir.z};
+        wray_init(&r, o, d, 1e-3f);
+        hit32 h;
+        trace_f32(s->bvh, &r, 1e16f, &h);
+        if (h.prim == 0xFFFFFFFFu) return 1;
+        dep++;
+        if (dep == maxd) return 0;
+        const uint32_t ti = s->tri_inst[h.prim];
+        const orc_thing* th = &s->things[ti];
+        const float* tri = s->tris + 9 * (size_t)h.prim;
+        f3 A = ld3(tri), B = ld3(tri + 3), C = ld3(tri + 6);
+        float u = h.u, v = h.v, w = 1.f - u - v;
+        f3 hit = add(add(scl(w, A), scl(u, B)), scl(v, C));
+        f3 N = unitv(cross(sub(B, A), sub(C, A)));
+        if (dot(dir, N) > 0.f) N = neg(N);
+        fvert* fv = &vs[(*nv)++];
+        fv->type = th->type; fv->thing = ti; fv->N = N; fv->ratio = 0.f; fv->refl = 0;
+        f3 nd;
+        if (th->type == ORC_DIFFUSE) {
+            fv->rv = rnd_on_sphere(rng);
+            nd = add(N, fv->rv);
+        } else if (th->type == ORC_REFLECT) {
+            f3 d1 = unitv(dir);
+            fv->rv = rnd_in_sphere(rng);
+            nd = add(reflect_unit(d1, N), scl(th->fuzz, fv->rv));
+            if (!(dot(nd, N) > 0)) return 0;
+        } else {
+            f3 d1 = unitv(dir);
+            float cos_t = fminf(dot(neg(d1), N), 1.f);
+            float sin_t = sqrtf(1.f - cos_t * cos_t);
+            f3 O = sub(hit, ld3(th->center));
+            float ratio = 0.f > dot(dir, O) ? 1.f / th->index : th->index;
+            int cannot = ratio * sin_t > 1.f;
+            float schlick = schlick_f(cos_t, ratio);
+            fv->ratio = ratio;
+            if (cannot || schlick > orc_frand48_next(rng)) { fv->refl = 1; nd = reflect_unit(d1, N); }
+            else nd = refract_unit(d1, N, ratio);
+        }
+        ori = hit;
+        dir = nd;
+    }
+}
+
+static void frozen_eval(const fvert* vs, int nv, f3 dir0, const float* params, const float amb[3], double out[3]) {
+    double d[3] = {dir0.x, dir0.y, dir0.z}, thr[3] = {1, 1, 1};
+    for (int k = 0; k < nv; k++) {
+        const fvert* v = &vs[k];
+        const float* p = params + 4 * (size_t)v->thing;
+        const double N[3] = {v->N.x, v->N.y, v->N.z};
+        if (v->type == ORC_DIFFUSE) {
+            for (int c = 0; c < 3; c++) thr[c] *= p[c];
+            d[0] = N[0] + v->rv.x; d[1] = N[1] + v->rv.y; d[2] = N[2] + v->rv.z;
+            continue;
+        }
+        const double il = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double u[3] = {d[0] * il, d[1] * il, d[2] * il};
+        const double un = u[0] * N[0] + u[1] * N[1] + u[2] * N[2];
+        if (v->type == ORC_REFLECT) {
+            const double rv[3] = {v->rv.x, v->rv.y, v->rv.z};
+            for (int c = 0; c < 3; c++) { d[c] = u[c] - 2 * un * N[c] + (double)p[3] * rv[c]; thr[c] *= p[c]; }
+        } else if (v->refl) {
+            for (int c = 0; c < 3; c++) d[c] = u[c] - 2 * un * N[c];
+        } else {
+            const double th = fmin(-un, 1.0), ratio = v->ratio;
+            double pp[3], q = 1;
+            for (int c = 0; c < 3; c++) { pp[c] = ratio * (u[c] + th * N[c]); q -= pp[c] * pp[c]; }
+            const double sq = sqrt(fabs(q));
+            for (int c = 0; c < 3; c++) d[c] = pp[c] - sq * N[c];
+        }
+    }
+    const double il = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double t = .5 * (d[1] * il + 1.0);
+    for (int c = 0; c < 3; c++) out[c] = thr[c] * ((1.0 - t) + t * amb[c]);
+}
+
+typedef struct {
+    const orc_scene* s; const orc_camera* cam;
+    uint32_t w, h, spp, depth, y0, y1;
+    const float* params;
+    double* raw;
+    int tid, nthreads;
+} fjob;
+
+static void* frozen_worker(void* arg) {
+    fjob* j = (fjob*)arg;
+    fvert vs[64];
+    for (uint32_t y = j->y0 + (uint32_t)j->tid; y < j->y1; y += (uint32_t)j->nthreads) {
+        for (uint32_t x = 0; x < j->w; x++) {
+            const uint32_t pix = j->w * y + x;
+            orc_frand48 rng;
+            orc_frand48_init(&rng, pix);
+            double acc[3] = {0, 0, 0};
+            for (uint32_t i = 0; i < j->spp; i++) {
+                float s, t;
+                pixel_st(x, y, j->w, j->h, &rng, &s, &t);
+                float o[3], d[3];
+                orc_camera_ray(j->cam, s, t, &rng, o, d);
+                int nv = 0;
+                if (frozen_trace(j->s, j->depth, &rng, ld3(o), ld3(d), vs, &nv)) {
+                    double col[3];
+                    frozen_eval(vs, nv, ld3(d), j->params, j->s->ambient, col);
+                    for (int c = 0; c < 3; c++) acc[c] += col[c];
+                }
+            }
+            for (int c = 0; c < 3; c++) {
+                const double m = acc[c] / j->spp;
+                j->raw[3 * (size_t)pix + c] = m < 0 ? 0 : m > 1 ? 1 : m;
+            }
+        }
+    }
+    return NULL;
+}
+
+void orc_render_frozen(const orc_scene* s, const orc_camera* cam, uint32_t w, uint32_t h, uint32_t spp,
+                       uint32_t depth, uint32_t y0, uint32_t y1, int nthreads, const float* params, double* raw) {
+    if (nthreads <= 0) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (y1 > h) y1 = h;
+    if (depth > 64) depth = 64;
+    pthread_t th[256];
+    fjob jobs[256];
+    for (int i = 0; i < nthreads; i++) {
+        fjob jb = {s, cam, w, h, spp, depth, y0, y1, params, raw, i, nthreads};
+        jobs[i] = jb;
+    }
+    for (int i = 1; i < nthreads; i++) pthread_create(&th[i], NULL, frozen_worker, &jobs[i]);
+    frozen_worker(&jobs[0]);
+    for (int i = 1; i < nthreads; i++) pthread_join(th[i], NULL);
+}
+
+/* ========================================================================= */
 /* Restatement of rtow.cxx (FP64, analytic spheres, glibc rand()).            */
 /* Operation order follows the reference expressions one for one:            */
 /* v.h, ray.h, util.h, sphere.h, things.h, camera.h, optics.h, rtow.cxx.       */

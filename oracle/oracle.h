@@ -27,6 +27,10 @@
  *                        statistically against orc_rtow_render-style analytic
  *                        scenes; parity of the whole mesh render is "statistical".
  *   orc_pp_srgb          IEC 61966-2-1 curve evaluated in FP64.
+ *   orc_render_frozen    equals orc_render at the scene's own materials (to FP32
+ *                        rounding); its albedo dependence is a product (checked
+ *                        by exact linearity in one channel) and its FD in fuzz
+ *                        is the checker of the adjoint kernel's fuzz gradient.
  */
 #ifndef RTX_ORACLE_H
 #define RTX_ORACLE_H
@@ -133,6 +137,17 @@ uint64_t orc_render(const orc_scene* s, const orc_camera* cam, uint32_t w, uint3
                     uint32_t spp, uint32_t depth, uint32_t y0, uint32_t y1, int nthreads,
                     float* raw, uint32_t* rpp, float* normals, float* albedos, float* sumsq,
                     uint32_t seed_base);
+
+/* Frozen-structure render for finite differences of material parameters:
+ * each sample path is traced as orc_render traces it (the scene's own
+ * materials decide the path), then its radiance is evaluated in FP64 along
+ * the recorded vertices with the materials `params` (float4 per thing:
+ * albedo rgb, fuzz), without re-tracing: hits, normals, random vectors and
+ * discrete choices stay fixed.  With params equal to the scene's materials
+ * this is orc_render's image up to FP32-vs-FP64 rounding.  raw: 3*w*h
+ * clamped means (rows [y0,y1) written).  depth <= 64. */
+void orc_render_frozen(const orc_scene* s, const orc_camera* cam, uint32_t w, uint32_t h, uint32_t spp,
+                       uint32_t depth, uint32_t y0, uint32_t y1, int nthreads, const float* params, double* raw);
 
 /* ---- Restatement of rtow.cxx (FP64, analytic spheres, glibc rand) -------- */
 /* argorder: 0 = function arguments evaluated left to right, 1 = right to left

@@ -5,6 +5,7 @@ is no fallback path.
 """
 import ctypes
 import os
+import sys
 from ctypes import (POINTER, Structure, Union, byref, c_bool, c_char_p, c_float, c_int, c_int32,
                     c_size_t, c_uint8, c_uint32, c_uint64, c_void_p)
 
@@ -116,6 +117,9 @@ SIGNATURES = {
     "rtx_tuning_get": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_tuning_set": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_launch": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),
+    "rtx_scene_set_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
+    "rtx_scene_get_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
+    "rtx_launch_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "rtx_batch_create": (c_int, [c_void_p, c_void_p, c_uint32, c_int, POINTER(c_void_p)]),
     "rtx_batch_launch": (c_int, [c_void_p, c_void_p]),
     "rtx_batch_destroy": (c_int, [c_void_p]),
@@ -147,6 +151,7 @@ class RtxError(RuntimeError):
 
 
 _lib = None
+_after_torch = False
 
 
 def lib():
@@ -155,6 +160,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("librtx.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        global _after_torch
+        # with torch in the process, its bundled HIP runtime must be the one
+        # librtx binds to (one runtime, shared device pointers and streams):
+        # that holds when torch was imported first
+        _after_torch = "torch" in sys.modules
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -319,6 +319,31 @@ class Scene:
         return {k: getattr(t, k) for k in ("shade_batch", "inner_frac", "tile_w", "stack_lds", "blocks_per_cu",
                                            "max_blocks_per_cu")}
 
+    def set_materials(self, params, handle=None, stream=None):
+        """Material parameters float32[n,4] (albedo rgb, fuzz per thing) into the
+        built scene; params: a numpy array, or an int device pointer."""
+        h = c_uint64(handle or self.handle)
+        if isinstance(params, (int, np.integer)):
+            call("rtx_scene_set_materials", self.ctx, h, c_void_p(int(params)), _stream_ptr(stream))
+            return
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 4)
+        if len(p) != len(self):
+            raise RtxError(1, "set_materials: %d rows for %d things" % (len(p), len(self)))
+        d = DeviceBuffer.from_array(p)
+        call("rtx_scene_set_materials", self.ctx, h, d.ptr, _stream_ptr(stream))
+        synchronize()
+        d.free()
+
+    def get_materials(self, handle=None):
+        """float32[n,4] material parameters of the built scene (dielectric: 0,0,0,ior)."""
+        n = len(self)
+        d = DeviceBuffer(16 * max(n, 1))
+        call("rtx_scene_get_materials", self.ctx, c_uint64(handle or self.handle), d.ptr, None)
+        synchronize()
+        out = d.download(np.float32, (n, 4))
+        d.free()
+        return out
+
     def set_miss_color(self, ambient):
         call("rtx_set_miss_color", self.ctx, _f3(ambient))
 
@@ -388,6 +413,20 @@ class Launcher:
         dt.free()
         dc.free()
         return out + (cyc,) if cycles else out
+
+    def adjoint(self, dldi, stream=None):
+        """dL/d material params float32[n,4] given dL/d rawRGB float32[h,w,3];
+        the Launcher's rawRGB must hold the forward image of the same lp."""
+        g = np.ascontiguousarray(dldi, np.float32).reshape(self.lp.image_h, self.lp.image_w, 3)
+        n = len(self.scene)
+        dg = DeviceBuffer.from_array(g)
+        dp = DeviceBuffer(16 * max(n, 1))
+        call("rtx_launch_adjoint", self.scene.ctx, byref(self.lp), dg.ptr, dp.ptr, _stream_ptr(stream))
+        synchronize()
+        out = dp.download(np.float32, (n, 4))
+        dg.free()
+        dp.free()
+        return out
 
     def free(self):
         for b in self._bufs.values():

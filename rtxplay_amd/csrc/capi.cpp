@@ -103,6 +103,8 @@ struct rtx_ctx {
     Built* built = nullptr;
     rtx_float3 ambient{.5f, .7f, 1.f};
     rtx::LaunchScratch render_ls, trace_ls;
+    uint32_t* tape = nullptr;       // adjoint path records
+    uint32_t tape_depth = 0;        // depth the tape was sized for
 };
 
 extern "C" {
@@ -173,6 +175,7 @@ rtx_status rtx_ctx_destroy(rtx_ctx* c) {
         if (ls->counter) (void)hipFree(ls->counter);
         if (ls->gstack) (void)hipFree(ls->gstack);
     }
+    if (c->tape) (void)hipFree(c->tape);
     delete c;
     return RTX_SUCCESS;
 }
@@ -668,6 +671,72 @@ rtx_status rtx_batch_destroy(rtx_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     delete b;
+    return RTX_SUCCESS;
+}
+
+// ---- differentiable rendering ----------------------------------------------
+static Built* material_target(rtx_ctx* c, uint64_t h, const char* what) {
+    Built* b = lookup(h);
+    if (!b) {
+        fail(RTX_ERROR_INVALID_HANDLE, std::string(what) + ": unknown handle");
+        return nullptr;
+    }
+    if (b->device != c->device) {
+        fail(RTX_ERROR_INVALID_HANDLE, std::string(what) + ": handle is on another device");
+        return nullptr;
+    }
+    return b;
+}
+
+rtx_status rtx_scene_set_materials(rtx_ctx* c, uint64_t is_handle, const float* params, void* stream) {
+    if (!c || !params) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_set_materials: NULL argument");
+    Built* b = material_target(c, is_handle, "rtx_scene_set_materials");
+    if (!b) return RTX_ERROR_INVALID_HANDLE;
+    HIP_OK(hipSetDevice(c->device));
+    hipError_t e = rtx::launch_set_materials(b->hdr, params, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_scene_set_materials");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_scene_get_materials(rtx_ctx* c, uint64_t is_handle, float* params, void* stream) {
+    if (!c || !params) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_get_materials: NULL argument");
+    Built* b = material_target(c, is_handle, "rtx_scene_get_materials");
+    if (!b) return RTX_ERROR_INVALID_HANDLE;
+    HIP_OK(hipSetDevice(c->device));
+    hipError_t e = rtx::launch_get_materials(b->hdr, params, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_scene_get_materials");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_launch_adjoint(rtx_ctx* c, const rtx_lp_general* lp, const rtx_float3* dL_drgb, float* d_params,
+                              void* stream) {
+    if (!c || !lp || !dL_drgb || !d_params) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_adjoint: NULL argument");
+    Built* b = material_target(c, lp->is_handle, "rtx_launch_adjoint");
+    if (!b) return RTX_ERROR_INVALID_HANDLE;
+    if (lp->image_w == 0 || lp->image_h == 0 || lp->spp == 0 || lp->depth == 0 || !lp->rawRGB)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_adjoint: bad launch parameters");
+    if ((uint64_t)lp->image_w * lp->image_h > 0xFFFFFFFFull / 2)
+        return fail(RTX_ERROR_LIMIT, "rtx_launch_adjoint: image too large");
+    if (lp->depth > 64) return fail(RTX_ERROR_LIMIT, "rtx_launch_adjoint: depth > 64");
+    if (b->hdr.num_insts >= (1u << 24)) return fail(RTX_ERROR_LIMIT, "rtx_launch_adjoint: 2^24 or more instances");
+    HIP_OK(hipSetDevice(c->device));
+    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    if (c->tape_depth < lp->depth) {
+        // (depth - 1) path records of 14 words for every lane of the grid
+        if (c->tape) {
+            HIP_OK(hipDeviceSynchronize());
+            HIP_OK(hipFree(c->tape));
+            c->tape = nullptr;
+            c->tape_depth = 0;
+        }
+        const size_t words = (size_t)(lp->depth > 1 ? lp->depth - 1 : 1) * rtx::kAdjointTapeWords * c->render_ls.gstride;
+        HIP_OK(hipMalloc((void**)&c->tape, words * sizeof(uint32_t)));
+        c->tape_depth = lp->depth;
+    }
+    const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
+    hipError_t e = rtx::launch_adjoint(b->hdr, *lp, amb, c->render_ls, dL_drgb, d_params, c->tape,
+                                       c->render_ls.gstride, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rtx_launch_adjoint");
     return RTX_SUCCESS;
 }
 

@@ -51,6 +51,15 @@ hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, f
 // all jobs of a batch in one persistent launch (d_jobs sorted by work_begin)
 hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njobs, uint32_t nwork,
                                const LaunchScratch& ls, uint32_t tile_w, hipStream_t s);
+// adjoint of a frame: dparams (float4 per instance: d albedo rgb, d fuzz) =
+// dL/d params given dL/d rawRGB (lp.rawRGB holds the forward image); tape:
+// (depth - 1) x kAdjointTapeWords words per lane, lane stride tape_stride
+constexpr uint32_t kAdjointTapeWords = 14;
+constexpr uint32_t kAdjointLdsInsts = 2048;   // instances whose gradients a block sums in LDS first
+hipError_t launch_adjoint(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,
+                          const rtx_float3* dldi, float* dparams, uint32_t* tape, uint32_t tape_stride, hipStream_t s);
+hipError_t launch_set_materials(const DevScene& sc, const float* params, hipStream_t s);
+hipError_t launch_get_materials(const DevScene& sc, float* params, hipStream_t s);
 hipError_t launch_pick(const DevScene& sc, const rtx_lp_general& lp, const LaunchScratch& ls, hipStream_t s);
 hipError_t launch_trace(const DevScene& sc, const rtx_ray* rays, uint32_t n, rtx_hit* hits,
                         const LaunchScratch& ls, hipStream_t s);

@@ -43,7 +43,21 @@ struct RenderArgs {
     uint32_t inner_frac;  // node phase ends below inner_frac/64 of the going lanes
     uint32_t tile_log2w;  // tile width = 1 << tile_log2w, tile height = 64 / width
     TraceCounters tc;
+    // adjoint launch only
+    const rtx_float3* dldi;   // dL/d rawRGB per pixel
+    float* dparams;           // float4 per instance: d albedo (rgb), d fuzz -- accumulated
+    uint32_t* tape;           // per-lane path records, word w of slot k at tape[(k*kTapeWords+w)*tape_stride + lane]
+    uint32_t tape_stride;     // lanes covered by the tape (grid x block)
+    uint32_t lds_grad;        // 1: accumulate per block in LDS first (num_insts x 4 floats)
 };
+
+// path record of the adjoint: meta (instance | type << 24 | flags), the
+// throughput before the vertex (3), the unit incoming direction (3), 1/|d|,
+// the facing normal (3), the metal's sphere sample r (3) or the dielectric's
+// ratio
+constexpr uint32_t kTapeWords = kAdjointTapeWords;
+constexpr uint32_t kTapeRefract = 1u << 26;   // dielectric: refracted (else reflected)
+constexpr uint32_t kTapeCosClamp = 1u << 27;  // dielectric: cos theta clamped to 1
 
 __device__ __forceinline__ float3 ld(const rtx_float3& v) { return f3(v.x, v.y, v.z); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return lo > x ? lo : x > hi ? hi : x; }
@@ -65,8 +79,8 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
     return lo;
 }
 
-template <int BLOCK, bool COUNT, bool BATCH, int SLDS>
-__device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack) {
+template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false>
+__device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr) {
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = a.gstack + blockIdx.x * BLOCK + threadIdx.x;
@@ -92,6 +106,27 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
     uint64_t cyc_trav = 0;
     uint32_t wave_iters[2] = {0, 0};
     const uint64_t cyc_start = COUNT ? clock64() : 0;
+    // adjoint: per-pixel dL/d(sample radiance), path vertex count, tape column
+    float3 delta = f3(0.f, 0.f, 0.f);
+    uint32_t kv = 0;
+    uint32_t* tape = ADJ ? a.tape + blockIdx.x * BLOCK + threadIdx.x : nullptr;
+    auto tput = [&](uint32_t slot, uint32_t w, uint32_t v) {
+        tape[(size_t)(slot * kTapeWords + w) * a.tape_stride] = v;
+    };
+    auto tget = [&](uint32_t slot, uint32_t w) -> uint32_t {
+        return tape[(size_t)(slot * kTapeWords + w) * a.tape_stride];
+    };
+    auto tgetf = [&](uint32_t slot, uint32_t w) -> float { return __uint_as_float(tget(slot, w)); };
+    auto tget3 = [&](uint32_t slot, uint32_t w) -> float3 { return f3(tgetf(slot, w), tgetf(slot, w + 1), tgetf(slot, w + 2)); };
+    auto tput3 = [&](uint32_t slot, uint32_t w, float3 v) {
+        tput(slot, w, __float_as_uint(v.x));
+        tput(slot, w + 1, __float_as_uint(v.y));
+        tput(slot, w + 2, __float_as_uint(v.z));
+    };
+    auto grad = [&](uint32_t inst, uint32_t c, float v) {
+        if (a.lds_grad) atomicAdd(&s_grad[4 * inst + c], v);
+        else atomicAdd(&a.dparams[4 * inst + c], v);
+    };
 
     // optx/camera_r.cu:69-99 and optx/camera.h:67-77: the sample's primary ray
     auto start_sample = [&]() {
@@ -104,6 +139,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                    1e16f);
         going = true;
         k = 0;
+        kv = 0;
         thr = f3(1.f, 1.f, 1.f);
         gN = f3(0.f, 0.f, 0.f);
         gA = f3(0.f, 0.f, 0.f);
@@ -138,6 +174,15 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                     y = (tile / J->tiles_x) * (64u / tw) + (in >> a.tile_log2w);
                     if (x < W && y < H) {
                         active = true;
+                        if (ADJ) {
+                            // d loss / d sample radiance: through the clamp of the
+                            // mean (zero where it clamps) and the 1/spp of the mean
+                            const uint32_t pix = W * y + x;
+                            const float3 g = ld(a.dldi[pix]), im = ld(J->lp.rawRGB[pix]);
+                            const float is = 1.f / (float)spp;
+                            delta = f3(im.x > 0.f && im.x < 1.f ? g.x * is : 0.f, im.y > 0.f && im.y < 1.f ? g.y * is : 0.f,
+                                       im.z > 0.f && im.z < 1.f ? g.z * is : 0.f);
+                        }
                         rng.init(W * y + x);              // optx/camera_r.cu:35, :42-43
                         sample = 0;
                         rpp = 0;
@@ -166,8 +211,58 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             // __miss__ambient, optx/camera_r.cu:135-146
             const float3 unit = unitv(d);
             const float t = .5f * (unit.y + 1.f);
-            col = thr * ((1.f - t) * f3(1.f, 1.f, 1.f) + t * J->ambient);
+            const float3 sky = (1.f - t) * f3(1.f, 1.f, 1.f) + t * J->ambient;
+            col = thr * sky;
             end = true;
+            if (ADJ) {
+                // adjoint of the escaped path: radiance = (prod of albedos) * sky(unit(d)).
+                // Walk the path records backwards with S = the product of the
+                // factors after the vertex (sky included) and lam = dL/d(outgoing
+                // direction), which flows back through metal and dielectric
+                // vertices up to the last diffuse one.
+                const float3 amb = J->ambient;
+                const float gam = .5f * (delta.x * thr.x * (amb.x - 1.f) + delta.y * thr.y * (amb.y - 1.f) +
+                                         delta.z * thr.z * (amb.z - 1.f));
+                const float il = 1.f / sqrtf(dot(d, d));
+                float3 lam = (gam * il) * (f3(0.f, 1.f, 0.f) - unit.y * unit);
+                float3 S = sky;
+                bool chain = true;
+                for (int j = (int)kv - 1; j >= 0; --j) {
+                    const uint32_t meta = tget(j, 0);
+                    const uint32_t inst = meta & 0xFFFFFFu;
+                    const int type = (int)((meta >> 24) & 3u);
+                    if (type != RTX_OPTICS_REFRACT) {
+                        const float3 P = tget3(j, 1);
+                        const float3 alb = xyz(sc.insts[2 * inst]);
+                        const float3 gA = delta * P * S;
+                        grad(inst, 0, gA.x);
+                        grad(inst, 1, gA.y);
+                        grad(inst, 2, gA.z);
+                        S = S * alb;
+                        if (type == RTX_OPTICS_DIFFUSE) chain = false;
+                    }
+                    if (!chain) continue;
+                    const float3 u = tget3(j, 4);
+                    const float iln = tgetf(j, 7);
+                    const float3 N = tget3(j, 8);
+                    float3 lu;                                  // dL/d(unit incoming direction)
+                    if (type == RTX_OPTICS_REFLECT) {
+                        grad(inst, 3, dot(lam, tget3(j, 11)));
+                        lu = lam - (2.f * dot(lam, N)) * N;
+                    } else if (!(meta & kTapeRefract)) {
+                        lu = lam - (2.f * dot(lam, N)) * N;
+                    } else {
+                        const float ratio = tgetf(j, 11);
+                        const float c = (meta & kTapeCosClamp) ? 1.f : -dot(u, N);
+                        const float3 pp = ratio * (u + c * N);
+                        const float q = 1.f - dot(pp, pp);
+                        const float sq = sqrtf(fabsf(q));
+                        const float3 lpp = lam + ((q < 0.f ? -1.f : 1.f) / sq * dot(N, lam)) * pp;
+                        lu = (meta & kTapeCosClamp) ? ratio * lpp : ratio * (lpp - dot(N, lpp) * N);
+                    }
+                    lam = iln * (lu - dot(u, lu) * u);
+                }
+            }
         } else if (k == maxd) {
             end = true;                                    // optx/optics_r.cu:34-42
         } else {
@@ -183,6 +278,8 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             if (dot(d, N) > 0.f) N = -N;
             const int type = __float_as_int(m1.w);
             float3 nd;
+            uint32_t meta = inst | ((uint32_t)type << 24);
+            if (ADJ) tput3(kv, 1, thr);
             if (type == RTX_OPTICS_DIFFUSE) {
                 // optx/optics_r.cu:108, :135-146
                 nd = N + unitv(rnd_in_sphere(rng));
@@ -194,7 +291,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 // optx/optics_r.cu:235-283
                 const float3 d1 = unitv(d);
                 const float3 refl = d1 - (2.f * dot(d1, N)) * N;
-                nd = refl + m0.w * rnd_in_sphere(rng);
+                const float3 rs = rnd_in_sphere(rng);
+                nd = refl + m0.w * rs;
+                if (ADJ) tput3(kv, 11, rs);
                 const float3 alb = xyz(m0);
                 gN = N;
                 gA = alb;
@@ -218,7 +317,19 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                     const float3 perpen = ratio * (d1 + theta * N);
                     const float3 parall = (-sqrtf(fabsf(1.f - dot(perpen, perpen)))) * N;
                     nd = perpen + parall;
+                    if (ADJ) meta |= kTapeRefract | (dot(-d1, N) >= 1.f ? kTapeCosClamp : 0u);
                 }
+                if (ADJ) tput(kv, 11, __float_as_uint(ratio));
+            }
+            if (ADJ) {
+                tput(kv, 0, meta);
+                if (type != RTX_OPTICS_DIFFUSE) {
+                    const float il = 1.f / sqrtf(dot(d, d));
+                    tput3(kv, 4, il * d);
+                    tput(kv, 7, __float_as_uint(il));
+                    tput3(kv, 8, N);
+                }
+                kv++;
             }
             if (!end) {
                 trav_begin(tv, sc, hp, nd, 1e-3f, 1e16f);   // optx/optics_r.cu:115-132
@@ -231,7 +342,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             rpp += k;
             nacc = nacc + gN;
             aacc = aacc + gA;
-            if (++sample == spp) {
+            if (++sample == spp && ADJ) {
+                active = false;
+            } else if (sample == spp) {
                 const rtx_lp_general& lp = J->lp;
                 const uint32_t pix = W * y + x;            // optx/camera_r.cu:111-120
                 const float inv_spp = 1.f / (float)spp;    // optx/v.h:44: 1/b * a
@@ -273,6 +386,52 @@ template <int BLOCK, int SLDS>
 __global__ __launch_bounds__(BLOCK) void k_render_batch(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
     render_body<BLOCK, false, true, SLDS>(a, s_stack);
+}
+
+// adjoint: replays every sample path of a frame (same per-pixel Frand48
+// streams, same traversal) and back-propagates dL/d rawRGB along each
+// escaped path into per-instance d albedo and d fuzz; per-block LDS sums
+// (when the instances fit) are flushed with one global atomic per entry
+template <int BLOCK, int SLDS>
+__global__ __launch_bounds__(BLOCK) void k_adjoint(const RenderArgs a) {
+    __shared__ uint32_t s_stack[SLDS * BLOCK];
+    extern __shared__ float s_grad[];
+    const uint32_t ng = 4u * a.job.sc.num_insts;
+    if (a.lds_grad) {
+        for (uint32_t i = threadIdx.x; i < ng; i += BLOCK) s_grad[i] = 0.f;
+        __syncthreads();
+    }
+    render_body<BLOCK, false, false, SLDS, true>(a, s_stack, s_grad);
+    if (a.lds_grad) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < ng; i += BLOCK) {
+            const float v = s_grad[i];
+            if (v != 0.f) atomicAdd(&a.dparams[i], v);
+        }
+    }
+}
+
+__global__ void k_fill(float* p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// material parameters as float4 per instance: albedo (rgb) and fuzz; only
+// diffuse (albedo) and metal (albedo, fuzz) instances take them
+__global__ void k_set_materials(float4* insts, uint32_t n, const float4* __restrict__ params) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int type = __float_as_int(insts[2 * i + 1].w);
+    const float4 p = params[i];
+    float4 m = insts[2 * i];
+    if (type == RTX_OPTICS_DIFFUSE) m = make_float4(p.x, p.y, p.z, m.w);
+    else if (type == RTX_OPTICS_REFLECT) m = p;
+    insts[2 * i] = m;
+}
+
+__global__ void k_get_materials(const float4* __restrict__ insts, uint32_t n, float4* params) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) params[i] = insts[2 * i];
 }
 
 // picker: one primary ray through (pick_x, pick_y), optx/camera_r.cu:26-32 and
@@ -386,6 +545,43 @@ hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njo
         hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
     else
         hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+
+hipError_t launch_adjoint(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,
+                          const rtx_float3* dldi, float* dparams, uint32_t* tape, uint32_t tape_stride, hipStream_t s) {
+    RenderArgs a = base_args(ls);
+    a.job = make_job(sc, lp, ambient, 1u << a.tile_log2w, 0);
+    a.njobs = 1;
+    a.nwork = a.job.work_end;
+    a.dldi = dldi;
+    a.dparams = dparams;
+    a.tape = tape;
+    a.tape_stride = tape_stride;
+    a.lds_grad = sc.num_insts <= kAdjointLdsInsts ? 1u : 0u;
+    const uint32_t ng = 4u * sc.num_insts;
+    if (ng) hipLaunchKernelGGL(k_fill, dim3((ng + 255) / 256), dim3(256), 0, s, dparams, ng, 0.f);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
+    const size_t shm = a.lds_grad ? sizeof(float) * ng : 0;
+    if (ls.stack_lds == 16)
+        hipLaunchKernelGGL((k_adjoint<kRenderBlock, 16>), dim3(ls.grid), dim3(kRenderBlock), shm, s, a);
+    else
+        hipLaunchKernelGGL((k_adjoint<kRenderBlock, 32>), dim3(ls.grid), dim3(kRenderBlock), shm, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_materials(const DevScene& sc, const float* params, hipStream_t s) {
+    if (!sc.num_insts) return hipSuccess;
+    hipLaunchKernelGGL(k_set_materials, dim3((sc.num_insts + 255) / 256), dim3(256), 0, s, (float4*)sc.insts,
+                       sc.num_insts, (const float4*)params);
+    return hipGetLastError();
+}
+
+hipError_t launch_get_materials(const DevScene& sc, float* params, hipStream_t s) {
+    if (!sc.num_insts) return hipSuccess;
+    hipLaunchKernelGGL(k_get_materials, dim3((sc.num_insts + 255) / 256), dim3(256), 0, s, sc.insts, sc.num_insts,
+                       (float4*)params);
     return hipGetLastError();
 }
 

@@ -88,15 +88,17 @@ class RTOWScene(Scene):
     """RTWO::load (optx/rtwo.cxx:133-245) over unit-sphere meshes.
 
     level: an int (every sphere at that subdivision level) or "mixed" (the
-    reference's per-sphere levels)."""
+    reference's per-sphere levels); spec: a sphere_things-style list to load
+    instead of the seeded one."""
 
-    def __init__(self, seed=0, half=11, level=4, device=0):
+    def __init__(self, seed=0, half=11, level=4, device=0, spec=None):
         super().__init__(device)
         self.seed, self.half, self.level = seed, half, level
         self.things_spec = []
+        self._spec = spec
 
     def load(self):
-        spec = sphere_things(self.seed, self.half)
+        spec = self._spec if self._spec is not None else sphere_things(self.seed, self.half)
         levels = [self.level if not isinstance(self.level, str) else mixed_level(t, i, len(spec))
                   for i, t in enumerate(spec)]
         meshes = {}

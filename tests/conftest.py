@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+try:
+    # the differentiable harness hands torch tensors to librtx: torch must be
+    # loaded first so that both bind the same HIP runtime
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
