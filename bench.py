@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
+    ap.add_argument("--legs", default="frame,batch,diff",
+                    help="comma list of: frame (the headline render), batch (64 frames), diff (adjoint)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -230,8 +232,6 @@ def main():
         sys.exit(2)
 
     import rtxplay_amd as rtx
-    from rtxplay_amd.hip import Event, Stream
-    from rtxplay_amd.scene import RTOWScene
 
     level, w, h, spp, depth = WORKLOADS[args.workload]
     if rtx.device_count() < 1:
@@ -245,6 +245,50 @@ def main():
         rtx.synchronize()
         print(json.dumps({"copy_probe": {"pixels": n, "read_bytes": 12 * n, "write_bytes": 4 * n}}))
         return
+    legs = set(x for x in args.legs.split(",") if x)
+    if args.no_batch:
+        legs.discard("batch")
+    if args.no_diff:
+        legs.discard("diff")
+    fr = frame_leg(args, level, w, h, spp, depth) if "frame" in legs else None
+    batched = None
+    if "batch" in legs:
+        batched = batched_leg(args.batch_reps, args.retune, log)
+        log("batched speedup %.2fx" % batched["speedup"])
+    diff = diff_leg(max(3, args.batch_reps)) if "diff" in legs else None
+
+    cpu = None
+    if not args.no_cpu and fr is not None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(level, w, h, spp, depth, args.cpu_seconds, threads)
+        log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
+
+    fr = fr or {}
+    out = {
+        "metric": METRIC, "value": fr.get("mrays"), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fr.get("ms"), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "rtow-final-scene level-%d spheres (%s triangles), %dx%d, %d spp, depth %d, seed 0"
+                               % (level, fr.get("triangles"), w, h, spp, depth),
+                   "image": [w, h], "spp": spp, "depth": depth, "triangles": fr.get("triangles"),
+                   "parallelism": "single-gpu"},
+        "samples_per_s": fr.get("samples_per_s"), "Msamples_per_s": fr.get("Msamples_per_s"),
+        "rays_per_frame": fr.get("rays"), "rays_per_sample": fr.get("rays_per_sample"),
+        "bvh": fr.get("bvh"), "roofline": fr.get("roofline"),
+        "cpu_baseline": cpu,
+        "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
+        "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def frame_leg(args, level, w, h, spp, depth):
+    """The headline frame: build, warmup, K timed renders, then a statistics
+    launch at reduced spp for the per-ray node/triangle counts."""
+    import rtxplay_amd as rtx
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene
+
     t0 = time.perf_counter()
     scene = RTOWScene(seed=0, half=11, level=level)
     scene.load()
@@ -274,7 +318,7 @@ def main():
     e1.record(stream)
     e1.synchronize()
     wall = time.perf_counter() - wall0
-    ms = e0.elapsed_ms(e1) / args.steps
+    ms = e0.elapsed_ms(e1) / max(args.steps, 1)
     rpp = launcher.rays_per_pixel()
     rays = int(rpp.sum(dtype=np.uint64))
     log("timed %d steps: %.1f ms/step (wall %.2f s), %d rays/frame" % (args.steps, ms, wall, rays))
@@ -296,46 +340,17 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "render_%s_traffic.json" % args.workload)
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
-
     launcher.free()
     scene.close()
-
-    batched = None
-    if not args.no_batch:
-        batched = batched_leg(args.batch_reps, args.retune, log)
-        log("batched speedup %.2fx" % batched["speedup"])
-
-    diff = None
-    if not args.no_diff:
-        diff = diff_leg(max(3, args.batch_reps))
-
-    cpu = None
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(level, w, h, spp, depth, args.cpu_seconds, threads)
-        log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
-
-    out = {
-        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "rtow-final-scene level-%d spheres (%d triangles), %dx%d, %d spp, depth %d, seed 0"
-                               % (level, st["num_prims"], w, h, spp, depth),
-                   "image": [w, h], "spp": spp, "depth": depth, "triangles": st["num_prims"],
-                   "parallelism": "single-gpu"},
-        "samples_per_s": round(samples, 1), "Msamples_per_s": round(samples / 1e6, 2),
-        "rays_per_frame": rays, "rays_per_sample": round(rays / (w * h * spp), 4),
-        "bvh": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "max_depth": st["max_depth"],
-                "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 2),
-                     "tri_tests_per_ray": round(n_tri, 2)},
-        "cpu_baseline": cpu,
-        "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
-        "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
-    }
-    print(json.dumps(out), flush=True)
+    return {"mrays": round(mrays, 2), "ms": round(ms, 3), "triangles": st["num_prims"],
+            "samples_per_s": round(samples, 1), "Msamples_per_s": round(samples / 1e6, 2), "rays": rays,
+            "rays_per_sample": round(rays / (w * h * spp), 4),
+            "bvh": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "max_depth": st["max_depth"],
+                    "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1)},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                         "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 2),
+                         "tri_tests_per_ray": round(n_tri, 2)}}
 
 
 if __name__ == "__main__":
