@@ -150,6 +150,21 @@ class Scene {
     }
     rtx_ctx* context() { return ctx_; }
 
+    // launch tuning (no reference counterpart); fields left 0 keep their value
+    rtx_tuning tuning() {
+        rtx_tuning t{};
+        check(rtx_tuning_get(ctx_, &t), "rtx_tuning_get");
+        return t;
+    }
+    void tune(const rtx_tuning& t) { check(rtx_tuning_set(ctx_, &t), "rtx_tuning_set"); }
+    // material parameters, device float4 per thing (albedo rgb, fuzz)
+    void set_materials(TraversableHandle h, const float* d_params, void* stream = nullptr) {
+        check(rtx_scene_set_materials(ctx_, h, d_params, stream), "rtx_scene_set_materials");
+    }
+    void get_materials(TraversableHandle h, float* d_params, void* stream = nullptr) {
+        check(rtx_scene_get_materials(ctx_, h, d_params, stream), "rtx_scene_get_materials");
+    }
+
    private:
     rtx_ctx* ctx_ = nullptr;
     Thing at_{};
@@ -180,6 +195,13 @@ class Launcher {
         check(rtx_synchronize(), "rtx_synchronize");
     }
 
+    // dL/d material params (device float4 per thing) given dL/d rawRGB (device
+    // float3 per pixel); rawRGB must hold the forward image of this lp
+    void adjoint(const float3* d_dl_drgb, float* d_dparams, void* stream = nullptr) {
+        check(rtx_launch_adjoint(scene_.context(), &lp_, d_dl_drgb, d_dparams, stream), "rtx_launch_adjoint");
+        check(rtx_synchronize(), "rtx_synchronize");
+    }
+
    private:
     void alloc(void** p, size_t bytes) {
         check(rtx_malloc(p, bytes), "rtx_malloc");
@@ -196,6 +218,26 @@ class Launcher {
     LpGeneral& lp_;
     std::vector<void*> owned_;
     size_t npix_ = 0;
+};
+
+// Many frames in one persistent launch (no reference counterpart); frame i
+// equals Launcher::ignite of lps[i].  The first launch on a stream captures a
+// hipGraph that later launches replay.
+class Batch {
+   public:
+    Batch(Scene& scene, const LpGeneral* lps, unsigned int n, bool use_graph = true) {
+        check(rtx_batch_create(scene.context(), lps, n, use_graph ? 1 : 0, &b_), "rtx_batch_create");
+    }
+    ~Batch() noexcept(false) { check(rtx_batch_destroy(b_), "rtx_batch_destroy"); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    void launch(void* stream, bool sync = true) {
+        check(rtx_batch_launch(b_, stream), "rtx_batch_launch");
+        if (sync) check(rtx_synchronize(), "rtx_synchronize");
+    }
+
+   private:
+    rtx_batch* b_ = nullptr;
 };
 
 inline void pp_sRGB(const float3* src, uchar4* dst, int w, int h, void* stream = nullptr) {
