@@ -7,9 +7,10 @@ acceleration structure already built and resident in HBM.
 
 Prints ONE JSON line on stdout (progress goes to stderr).  value = traced rays
 per second (hits + misses, counted as the reference's rays-per-pixel AOV) in
-Mrays/s; samples/s beside it.  roofline: algorithmic bytes per ray (BVH node
-visits x 64 B + triangle tests x 48 B + shading fetch 80 B, counted by a
-statistics launch) x rays/s against the 8 TB/s HBM peak.  cpu_baseline: the
+Mrays/s; samples/s beside it.  roofline: algorithmic bytes per ray (4-wide
+node visits x 112 B + triangle tests x 48 B + shading fetch 80 B + ray in/out
+48 B, visits and tests counted by a statistics launch) x rays/s against the
+8 TB/s HBM peak.  cpu_baseline: the
 oracle's renderer (C, same scene, same semantics) on the host cores over a
 bounded sample of rows.
 
@@ -33,7 +34,10 @@ WORKLOADS = {
     "c2": (4, 1280, 720, 50, 16),
 }
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, SHADE_BYTES = 64, 48, 80
+# bytes per 4-wide node visit (6 plane float4 + the refs float4), per
+# triangle test, per shading step (hit triangle 48 + instance record 32) and
+# per ray for its own input/output (ray 32 in, hit 16 out)
+NODE_BYTES, TRI_BYTES, SHADE_BYTES, IO_BYTES = 112, 48, 80, 48
 
 
 def log(*a):
@@ -332,8 +336,7 @@ def frame_leg(args, level, w, h, spp, depth):
     lp.spp = spp
     n_node = float(nodes.sum(dtype=np.uint64)) / srays
     n_tri = float(tris.sum(dtype=np.uint64)) / srays
-    io = (12 + 4 + 12 + 12) * w * h / rays
-    b_ray = n_node * NODE_BYTES + n_tri * TRI_BYTES + SHADE_BYTES + io
+    b_ray = n_node * NODE_BYTES + n_tri * TRI_BYTES + SHADE_BYTES + IO_BYTES
     achieved = b_ray * rays / (ms * 1e-3) / 1e9
     log("per ray: %.1f node visits, %.1f triangle tests -> %.0f B/ray, %.0f GB/s" % (n_node, n_tri, b_ray, achieved))
     traffic = None

@@ -143,6 +143,8 @@ typedef struct {
     float    sah_cost;          /* SAH cost, C_trav = 1, C_isect = 1 */
     float    build_ms;          /* device time of the last build */
     uint64_t device_bytes;      /* nodes + leaf-ordered triangles + instance data */
+    uint32_t wide_nodes;        /* 4-wide nodes the traversal uses (collapsed from the binary tree) */
+    uint32_t wide_depth;        /* levels of the 4-wide tree */
 } rtx_scene_stats;
 rtx_status rtx_scene_stats_get(rtx_ctx* ctx, uint64_t is_handle, rtx_scene_stats* st);
 
@@ -154,11 +156,11 @@ rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, ui
 /* Launch tuning of a context (no reference counterpart).  shade_batch: in
  * the render kernel a wave traverses until at least this many of its lanes
  * hold a final closest hit, shades those and resumes the others (1..64; 64 =
- * every lane of the wave finishes first; default 16).  inner_frac: a
+ * every lane of the wave finishes first; default 32).  inner_frac: a
  * traversal round's node phase ends when fewer than inner_frac/64 of the
  * traversing lanes are still on inner nodes; the lanes on a leaf then test
  * it (1..64; 1 = every lane reaches a leaf first, 64 = the first lane that
- * reaches a leaf ends the phase; default 36).  tile_w: pixels are handed
+ * reaches a leaf ends the phase; default 30).  tile_w: pixels are handed
  * out in tiles of tile_w x (64 / tile_w) (1, 2, 4, ..., 64; default 8).
  * stack_lds: traversal stack entries per lane kept in LDS, the rest in a
  * global overflow area (16 or 32; default 32).  blocks_per_cu: persistent

@@ -87,7 +87,8 @@ class HitC(Structure):
 class SceneStatsC(Structure):
     _fields_ = [("num_prims", c_uint32), ("num_nodes", c_uint32), ("num_leaves", c_uint32),
                 ("max_depth", c_uint32), ("max_leaf", c_uint32), ("sah_cost", c_float),
-                ("build_ms", c_float), ("device_bytes", c_uint64)]
+                ("build_ms", c_float), ("device_bytes", c_uint64), ("wide_nodes", c_uint32),
+                ("wide_depth", c_uint32)]
 
 
 class TuningC(Structure):

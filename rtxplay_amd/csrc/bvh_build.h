@@ -27,10 +27,11 @@ struct InstBuild {
 };
 
 struct BuiltBVH {
-    float4* nodes = nullptr;    // 4 float4 per inner node (owned)
-    float4* tris = nullptr;     // 3 float4 per triangle, leaf order (owned)
+    float4* nodes = nullptr;    // kWideNodeF4 float4 per 4-wide node (owned)
+    float4* tris = nullptr;     // 3 float4 per triangle, leaf order, + 1 degenerate triangle at N (owned)
     uint32_t root = kEmptyRoot;
-    uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0, max_leaf = 0;
+    uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0, max_leaf = 0;   // of the binary SAH tree
+    uint32_t num_wide = 0, wide_depth = 0;                                    // of the 4-wide tree
     double sah = 0.0;
 };
 

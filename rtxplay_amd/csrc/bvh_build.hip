@@ -26,8 +26,12 @@ constexpr int kBins = 32;
 constexpr int kBuildBlock = 256;
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)
-constexpr float kCostTrav = 1.0f;
+#ifndef RTX_SAH_TRAV
+#define RTX_SAH_TRAV 1.0f
+#endif
+constexpr float kCostTrav = RTX_SAH_TRAV;
 constexpr float kCostIsect = 1.0f;
+constexpr float kFarAway = 1e38f;   // coordinate of the empty-slot boxes and the degenerate triangle
 
 struct BuildTask {
     uint32_t node, begin, end, depth;
@@ -458,30 +462,117 @@ __global__ void k_inner_flag(const BNode* __restrict__ bnodes, uint32_t nnodes, 
     if (i < nnodes) flag[i].kind = __float_as_uint(bnodes[i].hi.w) == 0u ? 1u : 0u;
 }
 
-__device__ __forceinline__ uint32_t child_ref(const BNode& c, uint32_t cidx, const uint32_t* tidx) {
-    const uint32_t cnt = __float_as_uint(c.hi.w);
-    return cnt ? leaf_ref(__float_as_uint(c.lo.w), cnt) : tidx[cidx];
+// ---------------------------------------------------------------------------
+// collapse of the binary tree into 4-wide nodes, level by level from the root.
+// A wide node starts from a binary node's two children and repeatedly opens
+// the inner child of largest surface area until it has 4 children or only
+// leaves; its inner children become the wide nodes of the next level.
+__global__ void k_collapse4(const BNode* __restrict__ bn, const uint32_t* __restrict__ front, uint32_t n,
+                            uint4* __restrict__ cid, uint32_t* __restrict__ ninner) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t L = __float_as_uint(bn[front[i]].lo.w);
+    uint32_t c0 = L, c1 = L + 1, c2 = kNoChild, c3 = kNoChild;
+    uint32_t cnt = 2;
+    for (int it = 0; it < 2; it++) {
+        // inner child of largest area among c0..c(cnt-1)
+        float best = -1.f;
+        int bk = -1;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t c = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
+            if ((uint32_t)k >= cnt) break;
+            const BNode b = bn[c];
+            if (__float_as_uint(b.hi.w) != 0u) continue;    // leaf
+            const float a = area3(b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z);
+            if (a > best) { best = a; bk = k; }
+        }
+        if (bk < 0) break;
+        const uint32_t opened = bk == 0 ? c0 : bk == 1 ? c1 : bk == 2 ? c2 : c3;
+        const uint32_t Lb = __float_as_uint(bn[opened].lo.w);
+        if (bk == 0) c0 = Lb; else if (bk == 1) c1 = Lb; else if (bk == 2) c2 = Lb; else c3 = Lb;
+        if (cnt == 2) c2 = Lb + 1; else c3 = Lb + 1;
+        cnt++;
+    }
+    uint32_t m = 0;
+    for (int k = 0; k < 4; k++) {
+        const uint32_t c = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
+        if (c != kNoChild && __float_as_uint(bn[c].hi.w) == 0u) m++;
+    }
+    cid[i] = make_uint4(c0, c1, c2, c3);
+    ninner[i] = m;
 }
 
-__global__ void k_convert(const BNode* __restrict__ bnodes, uint32_t nnodes, const uint32_t* __restrict__ tidx,
-                          float4* __restrict__ nodes) {
+__global__ void k_write4(const BNode* __restrict__ bn, const uint4* __restrict__ cid, const uint32_t* __restrict__ off,
+                         uint32_t n, uint32_t base_this, uint32_t base_next, uint32_t empty_ref,
+                         float4* __restrict__ nodes4, uint32_t* __restrict__ next_front) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nnodes) return;
-    const BNode nd = bnodes[i];
-    if (__float_as_uint(nd.hi.w) != 0u) return;   // leaves have no traversal node
-    const uint32_t L = __float_as_uint(nd.lo.w);
-    const BNode c0 = bnodes[L], c1 = bnodes[L + 1];
-    float4* o = nodes + 4 * (size_t)tidx[i];
-    o[0] = make_float4(c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y);
-    o[1] = make_float4(c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y);
-    o[2] = make_float4(c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z);
-    o[3] = make_float4(__uint_as_float(child_ref(c0, L, tidx)), __uint_as_float(child_ref(c1, L + 1, tidx)), 0.f, 0.f);
+    if (i >= n) return;
+    const uint4 c4 = cid[i];
+    float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
+    uint32_t ref[4];
+    uint32_t j = off[i];
+    for (int k = 0; k < 4; k++) {
+        const uint32_t c = k == 0 ? c4.x : k == 1 ? c4.y : k == 2 ? c4.z : c4.w;
+        if (c == kNoChild) {
+            // an empty slot: a point box far outside any scene, which a slab
+            // test accepts only for rays aimed at it with an unbounded tmax;
+            // its ref is a one-triangle leaf holding a degenerate triangle
+            // (no ray hits it), so even then the traversal stays in bounds
+            lx[k] = hx[k] = ly[k] = hy[k] = lz[k] = hz[k] = kFarAway;
+            ref[k] = empty_ref;
+            continue;
+        }
+        const BNode b = bn[c];
+        lx[k] = b.lo.x; hx[k] = b.hi.x; ly[k] = b.lo.y; hy[k] = b.hi.y; lz[k] = b.lo.z; hz[k] = b.hi.z;
+        const uint32_t cntp = __float_as_uint(b.hi.w);
+        if (cntp) {
+            ref[k] = leaf_ref(__float_as_uint(b.lo.w), cntp);
+        } else {
+            ref[k] = base_next + j;
+            next_front[j] = c;
+            j++;
+        }
+    }
+    float4* o = nodes4 + kWideNodeF4 * (size_t)(base_this + i);
+    o[0] = make_float4(lx[0], lx[1], lx[2], lx[3]);
+    o[1] = make_float4(hx[0], hx[1], hx[2], hx[3]);
+    o[2] = make_float4(ly[0], ly[1], ly[2], ly[3]);
+    o[3] = make_float4(hy[0], hy[1], hy[2], hy[3]);
+    o[4] = make_float4(lz[0], lz[1], lz[2], lz[3]);
+    o[5] = make_float4(hz[0], hz[1], hz[2], hz[3]);
+    o[6] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
+                       __uint_as_float(ref[3]));
+    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// single-block exclusive scan of n uints -> out, total -> *total
+__global__ __launch_bounds__(kScanBlock) void k_scan_u32(const uint32_t* __restrict__ in, uint32_t n,
+                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_ws[kScanBlock / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan<kScanBlock>(v, s_ws, tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
 }
 
 __global__ void k_gather_tris(const float4* __restrict__ tri_world, const uint32_t* __restrict__ idx, uint32_t N,
                               float4* __restrict__ tris) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
+    if (k == N) {
+        // the degenerate triangle behind the empty slots of 4-wide nodes
+        const float4 p = make_float4(kFarAway, kFarAway, kFarAway, __uint_as_float(kNoHit));
+        tris[3 * (size_t)N + 0] = p;
+        tris[3 * (size_t)N + 1] = make_float4(kFarAway, kFarAway, kFarAway, 0.f);
+        tris[3 * (size_t)N + 2] = make_float4(kFarAway, kFarAway, kFarAway, __uint_as_float(kNoHit));
+        return;
+    }
+    if (k > N) return;
     const size_t p = idx[k];
     tris[3 * (size_t)k + 0] = tri_world[3 * p + 0];
     tris[3 * (size_t)k + 1] = tri_world[3 * p + 1];
@@ -575,13 +666,49 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     RTX_TRY(hipStreamSynchronize(s));
 
     DevBuf<float4> nodes, tris;
-    RTX_TRY(nodes.alloc(4 * (size_t)(ninner ? ninner : 1)));
-    RTX_TRY(tris.alloc(3 * (size_t)N));
-    if (ninner)
-        hipLaunchKernelGGL(k_convert, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, rank.p,
-                           nodes.p);
-    hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(N, 256)), dim3(256), 0, s, d_tri_world, idx.p, N, tris.p);
+    RTX_TRY(tris.alloc(3 * ((size_t)N + 1)));
+    hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for((uint64_t)N + 1, 256)), dim3(256), 0, s, d_tri_world, idx.p, N,
+                       tris.p);
     RTX_TRY(hipGetLastError());
+
+    // 4-wide nodes: at most one per binary inner node
+    uint32_t nwide = 0, wide_depth = 0;
+    if (ninner) {
+        DevBuf<float4> wide;
+        DevBuf<uint32_t> fa, fb, cnt, off;
+        DevBuf<uint4> cid;
+        RTX_TRY(wide.alloc(kWideNodeF4 * (size_t)ninner));
+        RTX_TRY(fa.alloc(ninner));
+        RTX_TRY(fb.alloc(ninner));
+        RTX_TRY(cnt.alloc(ninner));
+        RTX_TRY(off.alloc(ninner));
+        RTX_TRY(cid.alloc(ninner));
+        const uint32_t zero = 0;
+        RTX_TRY(hipMemcpyAsync(fa.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, s));   // the root
+        uint32_t n = 1;
+        uint32_t* cur = fa.p;
+        uint32_t* nxt = fb.p;
+        while (n > 0) {
+            hipLaunchKernelGGL(k_collapse4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cur, n, cid.p,
+                               cnt.p);
+            hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, cnt.p, n, off.p, total.p);
+            hipLaunchKernelGGL(k_write4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cid.p, off.p, n,
+                               nwide, nwide + n, leaf_ref(N, 1), wide.p, nxt);
+            RTX_TRY(hipGetLastError());
+            uint32_t nn = 0;
+            RTX_TRY(hipMemcpyAsync(&nn, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            RTX_TRY(hipStreamSynchronize(s));
+            nwide += n;
+            wide_depth++;
+            n = nn;
+            std::swap(cur, nxt);
+        }
+        RTX_TRY(nodes.alloc(kWideNodeF4 * (size_t)nwide));
+        RTX_TRY(hipMemcpyAsync(nodes.p, wide.p, sizeof(float4) * kWideNodeF4 * (size_t)nwide,
+                               hipMemcpyDeviceToDevice, s));
+    } else {
+        RTX_TRY(nodes.alloc(kWideNodeF4));
+    }
     RTX_TRY(hipStreamSynchronize(s));
 
     const uint32_t root_cnt = __builtin_bit_cast(uint32_t, root.hi.w);
@@ -591,6 +718,8 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     out.tris = tris.release();
     out.root = root_cnt ? leaf_ref(__builtin_bit_cast(uint32_t, root.lo.w), root_cnt) : 0u;
     out.num_nodes = ninner;
+    out.num_wide = nwide;
+    out.wide_depth = wide_depth;
     out.num_leaves = hst.leaves;
     out.max_depth = hst.max_depth;
     out.max_leaf = hst.max_leaf;

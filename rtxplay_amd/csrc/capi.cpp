@@ -63,6 +63,7 @@ struct Built {
     float4* insts = nullptr;
     float4* tri_world = nullptr;       // original-order world triangles (3 float4 each)
     int device = 0;
+    uint32_t stack_need = 1;           // traversal stack entries per lane this scene can need
     rtx_scene_stats stats{};
     void release() {
         if (d_hdr) (void)hipFree(d_hdr);
@@ -121,11 +122,18 @@ static rtx_status alloc_scratch(rtx::LaunchScratch& ls, uint32_t blocks_per_cu, 
 
 // the traversal stack overflow area (tens of MB) is allocated by the first
 // launch that needs it, so contexts that only hold a scene for a batch cost
-// nothing beyond the scene
-static rtx_status ensure_gstack(rtx::LaunchScratch& ls) {
-    if (ls.gstack) return RTX_SUCCESS;
-    HIP_OK(hipMalloc((void**)&ls.gstack,
-                     sizeof(uint32_t) * (size_t)(rtx::kMaxTreeDepth - rtx::kMinStackLDS) * ls.gstride));
+// nothing beyond the scene; it grows when a scene needs a deeper stack
+static rtx_status ensure_gstack(rtx::LaunchScratch& ls, uint32_t need) {
+    const uint32_t over = need > (uint32_t)rtx::kMinStackLDS ? need - (uint32_t)rtx::kMinStackLDS : 1u;
+    if (ls.gstack && ls.gstack_entries >= over) return RTX_SUCCESS;
+    if (ls.gstack) {
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipFree(ls.gstack));
+        ls.gstack = nullptr;
+        ls.gstack_entries = 0;
+    }
+    HIP_OK(hipMalloc((void**)&ls.gstack, sizeof(uint32_t) * (size_t)over * ls.gstride));
+    ls.gstack_entries = over;
     return RTX_SUCCESS;
 }
 
@@ -458,6 +466,11 @@ static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if ((uint64_t)bvh.num_wide * 16u * rtx::kWideNodeF4 >= (1ull << 32) || ((uint64_t)N + 1) * 48u >= (1ull << 32)) {
+        b->release();
+        delete b;
+        return fail(RTX_ERROR_LIMIT, "rtx_scene_build: node or triangle array above 4 GiB (too many triangles)");
+    }
     if (bvh.max_depth > (uint32_t)rtx::kMaxTreeDepth) {
         b->release();
         delete b;
@@ -470,7 +483,11 @@ static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
     b->stats.max_leaf = bvh.max_leaf;
     b->stats.sah_cost = (float)bvh.sah;
     b->stats.build_ms = ms;
-    b->stats.device_bytes = sizeof(float4) * (4ull * bvh.num_nodes + 3ull * N + 2ull * ninst);
+    b->stats.device_bytes = sizeof(float4) * ((uint64_t)rtx::kWideNodeF4 * bvh.num_wide + 3ull * N + 2ull * ninst);
+    b->stats.wide_nodes = bvh.num_wide;
+    b->stats.wide_depth = bvh.wide_depth;
+    // a step pushes at most 3 entries per level of the 4-wide tree
+    b->stack_need = 3u * (bvh.wide_depth > 0 ? bvh.wide_depth : 1u);
     *out = b;
     return RTX_SUCCESS;
 }
@@ -555,7 +572,7 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
     if ((uint64_t)lp->image_w * lp->image_h > 0xFFFFFFFFull / 2)
         return fail(RTX_ERROR_LIMIT, "rtx_launch: image too large");
     HIP_OK(hipSetDevice(c->device));
-    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    if (rtx_status st = ensure_gstack(c->render_ls, b->stack_need)) return st;
     const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
     hipError_t e;
     if (once || lp->picker) {
@@ -584,7 +601,13 @@ struct rtx_batch {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    rtx::LaunchScratch captured{};    // the scratch and tuning the graph's kernel arguments hold
 };
+
+static bool same_scratch(const rtx::LaunchScratch& a, const rtx::LaunchScratch& b) {
+    return a.counter == b.counter && a.gstack == b.gstack && a.gstride == b.gstride && a.grid == b.grid &&
+           a.shade_batch == b.shade_batch && a.inner_frac == b.inner_frac && a.stack_lds == b.stack_lds;
+}
 
 extern "C" {
 
@@ -622,7 +645,9 @@ rtx_status rtx_batch_create(rtx_ctx* c, const rtx_lp_general* lps, uint32_t n, i
         if (work >= 0xFFFFFFFFull) return fail(RTX_ERROR_LIMIT, "rtx_batch_create: more than 2^32 pixel slots");
     }
     HIP_OK(hipSetDevice(c->device));
-    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    uint32_t need = 1;
+    for (uint32_t i = 0; i < n; i++) need = std::max(need, built[i]->stack_need);
+    if (rtx_status st = ensure_gstack(c->render_ls, need)) return st;
     rtx_batch* b = new rtx_batch();
     b->ctx = c;
     b->njobs = n;
@@ -650,7 +675,17 @@ rtx_status rtx_batch_launch(rtx_batch* b, void* stream) {
         if (e != hipSuccess) return hip_fail(e, "rtx_batch_launch");
         return RTX_SUCCESS;
     }
+    if (b->exec && !same_scratch(b->captured, b->ctx->render_ls)) {
+        // the context's scratch moved (a deeper scene grew the stack area) or its
+        // tuning changed since the capture: capture again
+        HIP_OK(hipStreamSynchronize(s));
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
     if (!b->exec) {
+        b->captured = b->ctx->render_ls;
         HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = rtx::launch_render_batch(b->d_jobs, b->first, b->njobs, b->nwork, b->ctx->render_ls, b->tile_w, s);
         hipGraph_t g = nullptr;
@@ -720,7 +755,7 @@ rtx_status rtx_launch_adjoint(rtx_ctx* c, const rtx_lp_general* lp, const rtx_fl
     if (lp->depth > 64) return fail(RTX_ERROR_LIMIT, "rtx_launch_adjoint: depth > 64");
     if (b->hdr.num_insts >= (1u << 24)) return fail(RTX_ERROR_LIMIT, "rtx_launch_adjoint: 2^24 or more instances");
     HIP_OK(hipSetDevice(c->device));
-    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    if (rtx_status st = ensure_gstack(c->render_ls, b->stack_need)) return st;
     if (c->tape_depth < lp->depth) {
         // (depth - 1) path records of 14 words for every lane of the grid
         if (c->tape) {
@@ -748,7 +783,7 @@ rtx_status rtx_launch_counted(rtx_ctx* c, const rtx_lp_general* lp, uint64_t* no
     if (lp->image_w == 0 || lp->image_h == 0 || lp->spp == 0 || lp->depth == 0 || !lp->rawRGB)
         return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_counted: bad launch parameters");
     HIP_OK(hipSetDevice(c->device));
-    if (rtx_status st = ensure_gstack(c->render_ls)) return st;
+    if (rtx_status st = ensure_gstack(c->render_ls, b->stack_need)) return st;
     rtx::TraceCounters tc;
     tc.nodes = node_visits;
     tc.tris = tri_tests;
@@ -765,7 +800,7 @@ rtx_status rtx_trace(rtx_ctx* c, uint64_t is_handle, const rtx_ray* rays, uint32
     if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_trace: unknown handle");
     if (n && (!rays || !hits)) return fail(RTX_ERROR_INVALID_VALUE, "rtx_trace: NULL buffer");
     HIP_OK(hipSetDevice(c->device));
-    if (rtx_status st = ensure_gstack(c->trace_ls)) return st;
+    if (rtx_status st = ensure_gstack(c->trace_ls, b->stack_need)) return st;
     hipError_t e = rtx::launch_trace(b->hdr, rays, n, hits, c->trace_ls, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "rtx_trace");
     return RTX_SUCCESS;

@@ -27,6 +27,12 @@
 #include "kernels.h"
 #include "traverse.h"
 
+// waves per SIMD the single-frame kernel is compiled for (register budget:
+// 4 waves leave 128 VGPRs, enough for the 4-wide node step without spills)
+#ifndef RTX_RENDER_WAVES
+#define RTX_RENDER_WAVES 4
+#endif
+
 namespace rtx {
 
 namespace {
@@ -373,10 +379,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
     }
 }
 
-// single frame: 96 VGPRs and (with 32 LDS stack entries) 32 KB of LDS per
-// block allow 5 waves per SIMD
+// single frame: one block of 256 lanes per SIMD wave slot
 template <int BLOCK, bool COUNT, int SLDS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_render(const RenderArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
     render_body<BLOCK, COUNT, false, SLDS>(a, s_stack);
 }
@@ -473,8 +478,8 @@ RenderArgs base_args(const LaunchScratch& ls) {
     a.counter = ls.counter;
     a.gstack = ls.gstack;
     a.gstride = ls.gstride;
-    a.batch = ls.shade_batch ? ls.shade_batch : 16u;
-    a.inner_frac = ls.inner_frac ? ls.inner_frac : 36u;
+    a.batch = ls.shade_batch ? ls.shade_batch : 32u;
+    a.inner_frac = ls.inner_frac ? ls.inner_frac : 30u;
     a.tile_log2w = log2u(ls.tile_w ? ls.tile_w : 8u);
     return a;
 }

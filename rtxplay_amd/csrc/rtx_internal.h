@@ -33,6 +33,8 @@ constexpr int kStackLDS = 32;             // traversal stack entries per lane ke
 constexpr int kMinStackLDS = 16;          // smallest LDS stack a render launch can be tuned to
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyRoot = 0xFFFFFFFFu;
+constexpr uint32_t kNoChild = 0x7FFFFFFFu;  // empty slot of a 4-wide node (no leaf flag, no valid node)
+constexpr int kWideNodeF4 = 8;             // float4 per 4-wide node (128 B)
 
 // What a traversable handle points at (device memory); kernels get a copy by value.
 struct DevScene {

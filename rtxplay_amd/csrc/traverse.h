@@ -1,11 +1,11 @@
 // traverse.h -- closest-hit query on the device: the work optixTrace does on
 // RT cores in the reference (optx/camera_r.cu:82-99, optx/optics_r.cu:115-132).
 //
-// Software traversal of the two-child BVH (layout in rtx_internal.h) with a
-// per-lane stack whose first kStackLDS entries live in LDS (one column per
-// lane, bank-conflict free) and the rest in a global overflow area.  The
-// builder bounds the tree depth by kMaxTreeDepth, so the stack never holds
-// more than kMaxTreeDepth - 1 entries.
+// Software traversal of the 4-wide BVH (layout in rtx_internal.h) with a
+// per-lane stack whose first SLDS entries live in LDS (one column per lane,
+// bank-conflict free) and the rest in a global overflow area.  A step pushes
+// at most 3 entries per level, so the stack holds at most 3 x (wide-tree
+// depth) entries; the launch sizes the overflow area for that.
 //
 // Triangles use the watertight test of Woop, Benthin and Wald (JCGT 2(1),
 // 2013): shared edges and vertices of a closed mesh let no ray through.  The
@@ -27,6 +27,10 @@ struct TRay {
     float tmin;
     int kx, ky, kz;
     float Sx, Sy, Sz;
+    // byte offsets, inside a 4-wide node, of the child planes the ray enters
+    // through on each axis (lo when the direction component is >= 0, else
+    // hi); the exit planes are at offset ^ 16
+    uint32_t nx, ny, nz;
 };
 
 __device__ __forceinline__ float comp(const float3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -49,6 +53,9 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     r.Sz = 1.f / dz;
     r.idir = f3(1.f / (d.x != 0.f ? d.x : 1e-30f), 1.f / (d.y != 0.f ? d.y : 1e-30f),
                 1.f / (d.z != 0.f ? d.z : 1e-30f));
+    r.nx = r.idir.x >= 0.f ? 0u : 16u;
+    r.ny = r.idir.y >= 0.f ? 32u : 48u;
+    r.nz = r.idir.z >= 0.f ? 64u : 80u;
 }
 
 // true with t > tmin and barycentrics (u,v) of v1 and v2 on a hit
@@ -116,7 +123,41 @@ struct LaneStack {
         if (i < SLDS) return lds[i * BLOCK];
         return glob[(size_t)(i - SLDS) * gstride];
     }
+    // One node step: returns the top entry (slot sp-1) and writes the hit
+    // children r1..r3 of a sorted 4-wide node above it, farthest lowest:
+    // child j (j = 1..nhit-1) goes to slot sp + nhit-1-j.  When every slot
+    // the step touches (sp-1 .. sp+2) is in LDS, the writes are unconditional
+    // and a child that is not pushed goes to the free slot sp+2 instead.
+    __device__ __forceinline__ uint32_t step4(int sp, int nhit, uint32_t r1, uint32_t r2, uint32_t r3) {
+        if (sp + 3 <= SLDS) {
+            const uint32_t top = lds[(sp > 0 ? sp - 1 : 0) * BLOCK];
+            const int s1 = nhit > 1 ? sp + nhit - 2 : sp + 2;
+            const int s2 = nhit > 2 ? sp + nhit - 3 : sp + 2;
+            const int s3 = nhit > 3 ? sp : sp + 2;
+            lds[s3 * BLOCK] = r3;
+            lds[s2 * BLOCK] = r2;
+            lds[s1 * BLOCK] = r1;
+            return top;
+        }
+        const uint32_t top = peek(sp);
+        if (nhit > 1) put(sp + nhit - 2, r1);
+        if (nhit > 2) put(sp + nhit - 3, r2);
+        if (nhit > 3) put(sp, r3);
+        return top;
+    }
 };
+
+__device__ __forceinline__ float comp4(const float4& v, int j) {
+    return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+}
+
+// compare-exchange of (key, ref) pairs: smaller key first
+__device__ __forceinline__ void sort_ce(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+    const bool sw = kb < ka;
+    const float k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const uint32_t r0 = sw ? rb : ra, r1 = sw ? ra : rb;
+    ka = k0; kb = k1; ra = r0; rb = r1;
+}
 
 struct HitFull {
     float t, u, v;
@@ -178,38 +219,54 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 (*nn)++;
                 if (first_active_lane()) wi[0]++;
             }
-            const float4* np = sc.nodes + 4 * (size_t)s.node;
-            const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            // one 4-wide node: six float4 of child bounds (SoA: lo.x, hi.x,
+            // lo.y, hi.y, lo.z, hi.z) + one of refs.  The ray loads each
+            // axis's entry and exit planes by its direction signs, so the
+            // slab test needs no min/max per axis.
+            // (32-bit byte offsets: the builder keeps the node array below 4 GiB)
+            const char* nb = (const char*)sc.nodes;
+            const uint32_t no = s.node * (16u * kWideNodeF4);
             const TRay& r = s.r;
-            const float a0 = (n0.x - r.o.x) * r.idir.x, a1 = (n0.y - r.o.x) * r.idir.x;
-            const float a2 = (n0.z - r.o.y) * r.idir.y, a3 = (n0.w - r.o.y) * r.idir.y;
-            const float a4 = (n2.x - r.o.z) * r.idir.z, a5 = (n2.y - r.o.z) * r.idir.z;
-            const float b0 = (n1.x - r.o.x) * r.idir.x, b1 = (n1.y - r.o.x) * r.idir.x;
-            const float b2 = (n1.z - r.o.y) * r.idir.y, b3 = (n1.w - r.o.y) * r.idir.y;
-            const float b4 = (n2.z - r.o.z) * r.idir.z, b5 = (n2.w - r.o.z) * r.idir.z;
-            const float c0min = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
-            const float c0max = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), s.h.t));
-            const float c1min = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
-            const float c1max = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), s.h.t));
-            const bool hit0 = c0min <= c0max * kBoxPad;
-            const bool hit1 = c1min <= c1max * kBoxPad;
-            // branch-free child order and stack update: the far child is
-            // written to the free slot every step and kept only if both
-            // children are hit; the top slot is read every step and used
-            // only if neither is
-            const bool both = hit0 && hit1;
-            const bool take1 = both ? (c1min < c0min) : hit1;
-            const uint32_t ca = __float_as_uint(n3.x), cb = __float_as_uint(n3.y);
-            const uint32_t nearc = take1 ? cb : ca, farc = take1 ? ca : cb;
-            st.put(s.sp, farc);
-            const uint32_t top = st.peek(s.sp);
-            const bool any = hit0 || hit1;
-            s.sp += both ? 1 : 0;
-            if (!any) {
+            const float4 ex = *(const float4*)(nb + (no + r.nx)), fx = *(const float4*)(nb + (no + (r.nx ^ 16u)));
+            const float4 ey = *(const float4*)(nb + (no + r.ny)), fy = *(const float4*)(nb + (no + (r.ny ^ 16u)));
+            const float4 ez = *(const float4*)(nb + (no + r.nz)), fz = *(const float4*)(nb + (no + (r.nz ^ 16u)));
+            const float4 rf = *(const float4*)(nb + (no + 96u));
+            float key[4];
+            uint32_t ref[4];
+            int nhit = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float x0 = (comp4(ex, j) - r.o.x) * r.idir.x, x1 = (comp4(fx, j) - r.o.x) * r.idir.x;
+                const float y0 = (comp4(ey, j) - r.o.y) * r.idir.y, y1 = (comp4(fy, j) - r.o.y) * r.idir.y;
+                const float z0 = (comp4(ez, j) - r.o.z) * r.idir.z, z1 = (comp4(fz, j) - r.o.z) * r.idir.z;
+                const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
+                const float tf = fminf(fminf(x1, y1), fminf(z1, s.h.t));
+                // empty slots hold a point box at 1e38 that no slab test accepts
+                const uint32_t c = __float_as_uint(comp4(rf, j));
+                const bool hit = tn <= tf * kBoxPad;
+                key[j] = hit ? tn : __uint_as_float(0x7F800000u);
+                ref[j] = c;
+                nhit += hit ? 1 : 0;
+            }
+            // ascending by entry distance (misses last): 5 compare-exchanges
+            sort_ce(key[0], ref[0], key[1], ref[1]);
+            sort_ce(key[2], ref[2], key[3], ref[3]);
+            sort_ce(key[0], ref[0], key[2], ref[2]);
+            sort_ce(key[1], ref[1], key[3], ref[3]);
+            sort_ce(key[1], ref[1], key[2], ref[2]);
+            // visit the nearest; push the others farthest first (slot
+            // sp + nhit-1-j for child j), so the next nearest is on top.
+            // With no hit the top of the stack is next (read before any
+            // write: the writes go to slots >= sp).
+            const uint32_t top = st.step4(s.sp, nhit, ref[1], ref[2], ref[3]);
+            if (nhit == 0) {
                 if (s.sp == 0) going = false;
                 s.sp = s.sp > 0 ? s.sp - 1 : 0;
+                s.node = top;
+            } else {
+                s.sp += nhit - 1;
+                s.node = ref[0];
             }
-            s.node = any ? nearc : top;
         }
         // leaf phase
         if (going && (s.node & kLeafFlag)) {
@@ -218,8 +275,11 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             if (COUNT) *nt += cnt;
             for (uint32_t k = first; k < first + cnt; k++) {
                 if (COUNT && first_active_lane()) wi[1]++;
-                const float4* tp = sc.tris + 3 * (size_t)k;
-                const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                // (32-bit byte offsets: the builder keeps the triangle array below 4 GiB)
+                const char* tb = (const char*)sc.tris;
+                const uint32_t to = k * 48u;
+                const float4 t0 = *(const float4*)(tb + to), t1 = *(const float4*)(tb + (to + 16u)),
+                             t2 = *(const float4*)(tb + (to + 32u));
                 float tt, uu, vv;
                 if (watertight(s.r, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv)) {
                     const uint32_t prim = __float_as_uint(t0.w);
