@@ -25,7 +25,10 @@ constexpr float kBoxPad = 1.0000004f;
 struct TRay {
     float3 o, d, idir;
     float tmin;
-    int kx, ky, kz;
+    // the watertight test's axis order (kx, ky, kz) as a rotation that puts
+    // the dominant axis kz last (rz0: kz = x, rz1: kz = y, else kz = z) and a
+    // swap of the first two when d[kz] < 0 (Woop et al. 2013, section 3)
+    bool rz0, rz1, swp;
     float Sx, Sy, Sz;
     // byte offsets, inside a 4-wide node, of the child planes the ray enters
     // through on each axis (lo when the direction component is >= 0, else
@@ -34,6 +37,14 @@ struct TRay {
 };
 
 __device__ __forceinline__ float comp(const float3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// (a[kx], a[ky], a[kz]) with selects only (no per-lane branches)
+__device__ __forceinline__ float3 permute(const TRay& r, float3 a) {
+    const float p = r.rz0 ? a.y : (r.rz1 ? a.z : a.x);
+    const float q = r.rz0 ? a.z : (r.rz1 ? a.x : a.y);
+    const float w = r.rz0 ? a.x : (r.rz1 ? a.y : a.z);
+    return r.swp ? f3(q, p, w) : f3(p, q, w);
+}
 
 __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmin) {
     r.o = o;
@@ -45,9 +56,9 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     int ky = kx == 2 ? 0 : kx + 1;
     const float dz = comp(d, kz);
     if (dz < 0.f) { const int t = kx; kx = ky; ky = t; }
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    r.rz0 = kz == 0;
+    r.rz1 = kz == 1;
+    r.swp = dz < 0.f;
     r.Sx = comp(d, kx) / dz;
     r.Sy = comp(d, ky) / dz;
     r.Sz = 1.f / dz;
@@ -65,11 +76,12 @@ __device__ __forceinline__ bool watertight(const TRay& r, float3 v0, float3 v1, 
     const float Ax0 = v0.x - r.o.x, Ay0 = v0.y - r.o.y, Az0 = v0.z - r.o.z;
     const float Bx0 = v1.x - r.o.x, By0 = v1.y - r.o.y, Bz0 = v1.z - r.o.z;
     const float Cx0 = v2.x - r.o.x, Cy0 = v2.y - r.o.y, Cz0 = v2.z - r.o.z;
-    const float3 A = f3(Ax0, Ay0, Az0), B = f3(Bx0, By0, Bz0), C = f3(Cx0, Cy0, Cz0);
-    const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-    const float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
-    const float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
-    const float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+    const float3 A = permute(r, f3(Ax0, Ay0, Az0)), B = permute(r, f3(Bx0, By0, Bz0)),
+                 C = permute(r, f3(Cx0, Cy0, Cz0));
+    const float Akz = A.z, Bkz = B.z, Ckz = C.z;
+    const float Ax = A.x - r.Sx * Akz, Ay = A.y - r.Sy * Akz;
+    const float Bx = B.x - r.Sx * Bkz, By = B.y - r.Sy * Bkz;
+    const float Cx = C.x - r.Sx * Ckz, Cy = C.y - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
