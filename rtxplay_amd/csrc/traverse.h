@@ -17,6 +17,8 @@
 
 #include "rtx_internal.h"
 
+
+
 namespace rtx {
 
 // slab test pad on the far distance: 1 + 2*gamma(3) (Ize 2013), conservative
@@ -36,7 +38,6 @@ struct TRay {
     uint32_t nx, ny, nz;
 };
 
-__device__ __forceinline__ float comp(const float3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 // (a[kx], a[ky], a[kz]) with selects only (no per-lane branches)
 __device__ __forceinline__ float3 permute(const TRay& r, float3 a) {
@@ -52,15 +53,13 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     r.tmin = tmin;
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     const int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
-    int kx = kz == 2 ? 0 : kz + 1;
-    int ky = kx == 2 ? 0 : kx + 1;
-    const float dz = comp(d, kz);
-    if (dz < 0.f) { const int t = kx; kx = ky; ky = t; }
+    const float dz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
     r.rz0 = kz == 0;
     r.rz1 = kz == 1;
     r.swp = dz < 0.f;
-    r.Sx = comp(d, kx) / dz;
-    r.Sy = comp(d, ky) / dz;
+    const float3 dp = permute(r, d);       // (d[kx], d[ky], d[kz])
+    r.Sx = dp.x / dz;
+    r.Sy = dp.y / dz;
     r.Sz = 1.f / dz;
     r.idir = f3(1.f / (d.x != 0.f ? d.x : 1e-30f), 1.f / (d.y != 0.f ? d.y : 1e-30f),
                 1.f / (d.z != 0.f ? d.z : 1e-30f));
@@ -182,7 +181,8 @@ struct HitFull {
 struct Trav {
     TRay r;
     HitFull h;
-    uint32_t node;
+    uint32_t node;    // next node to visit (kNoChild: none left)
+    uint32_t leaf;    // a leaf found but not tested yet (kNoChild: none)
     int sp;
 };
 
@@ -193,6 +193,7 @@ __device__ __forceinline__ void trav_begin(Trav& s, const DevScene& sc, float3 o
     s.h.tri = kNoHit;
     s.h.prim = kNoHit;
     s.node = sc.root;
+    s.leaf = kNoChild;
     s.sp = 0;
 }
 
@@ -206,10 +207,13 @@ __device__ __forceinline__ bool first_active_lane() {
 // `going` when its ray is done (closest hit final in s.h).
 //
 // Each round has a node phase and a leaf phase.  The node phase steps the
-// lanes that sit on an inner node and ends when fewer than inner_frac/64 of
-// the going lanes still do (lanes that reached a leaf wait for it); the leaf
-// phase tests the triangles of every lane on a leaf.  inner_frac = 1 gives
-// the classic while-while loop (all lanes reach a leaf first).  The function
+// lanes that sit on an inner node; a lane that reaches a leaf parks it in a
+// one-leaf slot and goes on with the next node from its stack (Aila and
+// Laine 2009, "speculative traversal"), and waits only when it reaches a
+// second leaf or runs out of nodes.  The phase ends when fewer than
+// inner_frac/64 of the going lanes can still step; the leaf phase then tests
+// the parked leaves of all lanes together.  inner_frac = 1 runs the node
+// phase until no lane can step.  The function
 // returns once no lane is going, or -- with `batch` < 64 -- as soon as at
 // least `batch` of the wave's `busy` lanes have finished, so that the caller
 // can shade them while the others keep their state and resume later.
@@ -222,7 +226,14 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
     for (;;) {
         // node phase
         for (;;) {
-            const bool inner = going && !(s.node & kLeafFlag);
+            // a lane that reached a leaf parks it (one slot) and goes on with
+            // the next node from its stack, so it keeps stepping while the
+            // others do; the parked leaves are tested together below
+            if (going && s.leaf == kNoChild && (s.node & kLeafFlag)) {
+                s.leaf = s.node;
+                s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
+            }
+            const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild;
             const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
             if (n_inner == 0) break;
             if (n_inner * 64u < (uint32_t)inner_frac * (uint32_t)__popcll(__ballot(going))) break;
@@ -272,18 +283,19 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             // write: the writes go to slots >= sp).
             const uint32_t top = st.step4(s.sp, nhit, ref[1], ref[2], ref[3]);
             if (nhit == 0) {
-                if (s.sp == 0) going = false;
+                s.node = s.sp == 0 ? kNoChild : top;
                 s.sp = s.sp > 0 ? s.sp - 1 : 0;
-                s.node = top;
             } else {
                 s.sp += nhit - 1;
                 s.node = ref[0];
             }
         }
         // leaf phase
-        if (going && (s.node & kLeafFlag)) {
-            const uint32_t first = s.node & kLeafFirstMask;
-            const uint32_t cnt = ((s.node >> kLeafCountShift) & 15u) + 1u;
+        if (going && s.leaf != kNoChild) {
+            const uint32_t lf = s.leaf;
+            s.leaf = kNoChild;
+            const uint32_t first = lf & kLeafFirstMask;
+            const uint32_t cnt = ((lf >> kLeafCountShift) & 15u) + 1u;
             if (COUNT) *nt += cnt;
             for (uint32_t k = first; k < first + cnt; k++) {
                 if (COUNT && first_active_lane()) wi[1]++;
@@ -304,9 +316,8 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                     }
                 }
             }
-            if (s.sp == 0) going = false;
-            else s.node = st.pop(s.sp);
         }
+        if (going && s.node == kNoChild && s.leaf == kNoChild) going = false;
         const uint64_t g = __ballot(going);
         if (g == 0) return;
         if (batch < 64 && __popcll(__ballot(busy && !going)) >= batch) return;
@@ -324,6 +335,7 @@ __device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, floa
     s.h.tri = kNoHit;
     s.h.prim = kNoHit;
     s.node = sc.root;
+    s.leaf = kNoChild;
     s.sp = 0;
     bool going = true;
     trav_run<BLOCK, COUNT>(sc, s, going, true, 64, 32, st, nn, nt);
