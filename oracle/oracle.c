@@ -613,9 +613,9 @@ static inline void wray_init(wray* r, const float o[3], const float d[3], float 
     int ky = kx + 1 == 3 ? 0 : kx + 1;
     if (d[kz] < 0.f) { int tmp = kx; kx = ky; ky = tmp; }
     r->kx = kx; r->ky = ky; r->kz = kz;
-    r->Sx = d[kx] / d[kz];
-    r->Sy = d[ky] / d[kz];
     r->Sz = 1.f / d[kz];
+    r->Sx = d[kx] * r->Sz;     /* shear by the reciprocal (one division per ray) */
+    r->Sy = d[ky] * r->Sz;
     for (int k = 0; k < 3; k++) r->inv[k] = 1.f / (d[k] != 0.f ? d[k] : 1e-30f);
     r->tmin = tmin;
 }
@@ -732,10 +732,12 @@ static inline f3 refract_unit(f3 d1, f3 N, float ratio) {
     f3 parall = scl(-sqrtf(fabsf(1.f - dot(perpen, perpen))), N);
     return add(perpen, parall);
 }
-/* optx/optics_r.cu:387-388 */
+/* optx/optics_r.cu:387-388; the fifth power as x^2 x^2 x (three roundings)
+ * where the reference calls powf */
 static inline float schlick_f(float cos_t, float ratio) {
     float r0 = (1.f - ratio) / (1.f + ratio); r0 = r0 * r0;
-    return r0 + (1.f - r0) * powf((1.f - cos_t), 5.f);
+    const float x = 1.f - cos_t, x2 = x * x;
+    return r0 + (1.f - r0) * ((x2 * x2) * x);
 }
 /* optx/camera_r.cu:135-140 */
 static inline f3 sky_f(f3 dir, f3 amb) {

@@ -315,7 +315,8 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 const bool cannot = ratio * sin_t > 1.f;
                 float r0 = (1.f - ratio) / (1.f + ratio);
                 r0 = r0 * r0;
-                const float schlick = r0 + (1.f - r0) * powf(1.f - cos_t, 5.f);
+                const float x = 1.f - cos_t, x2 = x * x;     // (1 - cos)^5 as x^2 x^2 x
+                const float schlick = r0 + (1.f - r0) * ((x2 * x2) * x);
                 if (cannot || schlick > rng.next()) {
                     nd = d1 - (2.f * dot(d1, N)) * N;
                 } else {

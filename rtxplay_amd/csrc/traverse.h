@@ -58,9 +58,9 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     r.rz1 = kz == 1;
     r.swp = dz < 0.f;
     const float3 dp = permute(r, d);       // (d[kx], d[ky], d[kz])
-    r.Sx = dp.x / dz;
-    r.Sy = dp.y / dz;
     r.Sz = 1.f / dz;
+    r.Sx = dp.x * r.Sz;                     // shear by the reciprocal: one division
+    r.Sy = dp.y * r.Sz;
     r.idir = f3(1.f / (d.x != 0.f ? d.x : 1e-30f), 1.f / (d.y != 0.f ? d.y : 1e-30f),
                 1.f / (d.z != 0.f ? d.z : 1e-30f));
     r.nx = r.idir.x >= 0.f ? 0u : 16u;
