@@ -286,18 +286,24 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             float3 nd;
             uint32_t meta = inst | ((uint32_t)type << 24);
             if (ADJ) tput3(kv, 1, thr);
-            if (type == RTX_OPTICS_DIFFUSE) {
+            // diffuse and metal lanes draw their point in the unit sphere in
+            // one rejection loop, and one unit() serves the diffuse sample
+            // and the incoming direction of metal and dielectric lanes (each
+            // lane's draws keep the reference order)
+            const bool diff = type == RTX_OPTICS_DIFFUSE, metal = type == RTX_OPTICS_REFLECT;
+            float3 rs = f3(0.f, 0.f, 0.f);
+            if (diff || metal) rs = rnd_in_sphere(rng);
+            const float3 un = unitv(diff ? rs : d);
+            const float3 refl = un - (2.f * dot(un, N)) * N;   // optx/optics_r.cu:236, :392
+            if (diff) {
                 // optx/optics_r.cu:108, :135-146
-                nd = N + unitv(rnd_in_sphere(rng));
+                nd = N + un;
                 const float3 alb = xyz(m0);
                 thr = thr * alb;
                 gN = N;
                 gA = alb;
-            } else if (type == RTX_OPTICS_REFLECT) {
+            } else if (metal) {
                 // optx/optics_r.cu:235-283
-                const float3 d1 = unitv(d);
-                const float3 refl = d1 - (2.f * dot(d1, N)) * N;
-                const float3 rs = rnd_in_sphere(rng);
                 nd = refl + m0.w * rs;
                 if (ADJ) tput3(kv, 11, rs);
                 const float3 alb = xyz(m0);
@@ -307,7 +313,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 else end = true;                           // absorbed: black
             } else {
                 // optx/optics_r.cu:372-398
-                const float3 d1 = unitv(d);
+                const float3 d1 = un;
                 const float cos_t = fminf(dot(-d1, N), 1.f);
                 const float sin_t = sqrtf(1.f - cos_t * cos_t);
                 const float3 O = hp - xyz(m1);             // center = instance translation
@@ -318,7 +324,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 const float x = 1.f - cos_t, x2 = x * x;     // (1 - cos)^5 as x^2 x^2 x
                 const float schlick = r0 + (1.f - r0) * ((x2 * x2) * x);
                 if (cannot || schlick > rng.next()) {
-                    nd = d1 - (2.f * dot(d1, N)) * N;
+                    nd = refl;
                 } else {
                     const float theta = fminf(dot(-d1, N), 1.f);
                     const float3 perpen = ratio * (d1 + theta * N);
