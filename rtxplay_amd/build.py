@@ -35,9 +35,13 @@ def hipcc():
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+# -ffp-contract=off: every kernel rounds each operation as written, so two
+# kernels that run the same arithmetic (a frame rendered alone or inside a
+# batch) produce the same bits whatever the compiler's scheduling, and the
+# oracle (built the same way) follows the same roundings
 DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics", "-ffp-contract=off"]
 HOST = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"]
 
 

@@ -134,22 +134,11 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         else atomicAdd(&a.dparams[4 * inst + c], v);
     };
 
-    // optx/camera_r.cu:69-99 and optx/camera.h:67-77: the sample's primary ray
-    auto start_sample = [&]() {
-        const rtx_camera& cam = J->lp.camera;
-        const float s = 2.f * ((float)x + rng.next()) / (float)W - 1.f;
-        const float t = 2.f * ((float)y + rng.next()) / (float)H - 1.f;
-        const float3 rd = (cam.aperture / 2.f) * rnd_in_disk(rng);
-        const float3 off = rd.x * ld(cam.u) + rd.y * ld(cam.v);
-        trav_begin(tv, sc, ld(cam.eye) + off, ((s * ld(cam.wvec) + t * ld(cam.hvec)) - ld(cam.dvec)) - off, 1e-3f,
-                   1e16f);
-        going = true;
-        k = 0;
-        kv = 0;
-        thr = f3(1.f, 1.f, 1.f);
-        gN = f3(0.f, 0.f, 0.f);
-        gA = f3(0.f, 0.f, 0.f);
-    };
+    // the next ray of a lane: a new sample's primary ray (newsample) or the
+    // scattered ray of a path that goes on (cont, from ro along rdv); both
+    // are set up at one place per iteration, so a wave pays for one setup
+    bool newsample = false, cont = false;
+    float3 ro = f3(0.f, 0.f, 0.f), rdv = f3(0.f, 0.f, 1.f);
 
     for (;;) {
         // lanes without a pixel take one; tile slots outside the frame are skipped
@@ -194,12 +183,33 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                         rpp = 0;
                         cnt_nodes = cnt_tris = 0;
                         acc = nacc = aacc = f3(0.f, 0.f, 0.f);
-                        start_sample();
+                        newsample = true;
                     }
                 }
             }
         }
         if (!__any(active)) break;
+
+        if (newsample) {
+            // optx/camera_r.cu:69-99 and optx/camera.h:67-77: the sample's primary ray
+            const rtx_camera& cam = J->lp.camera;
+            const float s = 2.f * ((float)x + rng.next()) / (float)W - 1.f;
+            const float t = 2.f * ((float)y + rng.next()) / (float)H - 1.f;
+            const float3 rd = (cam.aperture / 2.f) * rnd_in_disk(rng);
+            const float3 off = rd.x * ld(cam.u) + rd.y * ld(cam.v);
+            ro = ld(cam.eye) + off;
+            rdv = ((s * ld(cam.wvec) + t * ld(cam.hvec)) - ld(cam.dvec)) - off;
+            k = 0;
+            kv = 0;
+            thr = f3(1.f, 1.f, 1.f);
+            gN = f3(0.f, 0.f, 0.f);
+            gA = f3(0.f, 0.f, 0.f);
+        }
+        if (newsample || cont) {
+            trav_begin(tv, sc, ro, rdv, 1e-3f, 1e16f);     // optx/optics_r.cu:115-132 for a scattered ray
+            going = true;
+        }
+        newsample = cont = false;
 
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
@@ -345,8 +355,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 kv++;
             }
             if (!end) {
-                trav_begin(tv, sc, hp, nd, 1e-3f, 1e16f);   // optx/optics_r.cu:115-132
-                going = true;
+                ro = hp;
+                rdv = nd;
+                cont = true;
             }
         }
 
@@ -372,7 +383,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 }
                 active = false;
             } else {
-                start_sample();
+                newsample = true;
             }
         }
     }
