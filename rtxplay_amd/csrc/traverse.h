@@ -28,9 +28,11 @@ struct TRay {
     float3 o, d, idir;
     float tmin;
     // the watertight test's axis order (kx, ky, kz) as a rotation that puts
-    // the dominant axis kz last (rz0: kz = x, rz1: kz = y, else kz = z) and a
-    // swap of the first two when d[kz] < 0 (Woop et al. 2013, section 3)
-    bool rz0, rz1, swp;
+    // the dominant axis kz last (rz0: kz = x, rz1: kz = y, else kz = z).
+    // Woop et al. (2013) also swap kx and ky when d[kz] < 0 to keep the
+    // winding; that negates U, V, W, det and T alike and leaves t, u, v bit
+    // for bit the same, and nothing here depends on the winding
+    bool rz0, rz1;
     float Sx, Sy, Sz;
     // byte offsets, inside a 4-wide node, of the child planes the ray enters
     // through on each axis (lo when the direction component is >= 0, else
@@ -44,7 +46,7 @@ __device__ __forceinline__ float3 permute(const TRay& r, float3 a) {
     const float p = r.rz0 ? a.y : (r.rz1 ? a.z : a.x);
     const float q = r.rz0 ? a.z : (r.rz1 ? a.x : a.y);
     const float w = r.rz0 ? a.x : (r.rz1 ? a.y : a.z);
-    return r.swp ? f3(q, p, w) : f3(p, q, w);
+    return f3(p, q, w);
 }
 
 __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmin) {
@@ -56,7 +58,6 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     const float dz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
     r.rz0 = kz == 0;
     r.rz1 = kz == 1;
-    r.swp = dz < 0.f;
     const float3 dp = permute(r, d);       // (d[kx], d[ky], d[kz])
     r.Sz = 1.f / dz;
     r.Sx = dp.x * r.Sz;                     // shear by the reciprocal: one division
