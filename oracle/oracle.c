@@ -762,67 +762,71 @@ typedef struct {
     dgv_t* dgv;
 } tctx;
 
-/* one optixTrace: returns colour, updates *depth (payload 5) */
-static f3 trace_rec(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
+/* One sample path: the recursion of the closest-hit programs
+ * (optx/optics_r.cu:22-431 with the miss program optx/camera_r.cu:123-147)
+ * unrolled into a loop.  The reference multiplies the albedos on the way back
+ * up the recursion (col = alb * child); the loop multiplies them into a
+ * throughput on the way down, the order the GPU kernel uses (same factors,
+ * other association), so that both round alike.  The guide values are the
+ * deepest diffuse/metal hit's (the reference keeps the deepest non-zero one).
+ * Returns the colour; *depth = rays traced (payload 5). */
+static f3 trace_path(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
     const orc_scene* s = c->s;
-    wray r;
-    float o[3] = {ori.x, ori.y, ori.z}, d[3] = {dir.x, dir.y, dir.z};
-    wray_init(&r, o, d, 1e-3f);
-    hit32 h;
-    trace_f32(s->bvh, &r, 1e16f, &h);
-    if (h.prim == 0xFFFFFFFFu) {
-        /* __miss__ambient, optx/camera_r.cu:131-146 */
-        *depth = *depth + 1;
-        return sky_f(dir, ld3(s->ambient));
-    }
-    uint32_t dep = *depth + 1u;                        /* optx/optics_r.cu:31 */
-    if (c->depth == dep) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :34-42 */
-    const orc_thing* th = &s->things[s->tri_inst[h.prim]];
-    const float* tri = s->tris + 9 * (size_t)h.prim;
-    f3 A = ld3(tri), B = ld3(tri + 3), C = ld3(tri + 6);
-    float u = h.u, v = h.v, w = 1.f - u - v;          /* :78-81 */
-    f3 hit = add(add(scl(w, A), scl(u, B)), scl(v, C)); /* :84 */
-    f3 N = unitv(cross(sub(B, A), sub(C, A)));         /* :88 */
-    if (dot(dir, N) > 0.f) N = neg(N);                 /* :89-90 */
-    f3 col;
-    if (th->type == ORC_DIFFUSE) {
-        f3 nd = add(N, rnd_on_sphere(c->rng));         /* :108 */
-        f3 child = trace_rec(c, hit, nd, &dep);
-        f3 alb = ld3(th->albedo);
-        col = mul(alb, child);                         /* :135-138 */
-        if (isnull3(c->dgv->normal)) c->dgv->normal = N;    /* :143-146 */
-        if (isnull3(c->dgv->albedo)) c->dgv->albedo = alb;
-    } else if (th->type == ORC_REFLECT) {
-        f3 d1 = unitv(dir);                            /* :235 */
-        f3 refl = reflect_unit(d1, N);                 /* :236 */
-        f3 nd = add(refl, scl(th->fuzz, rnd_in_sphere(c->rng)));  /* :238 */
-        f3 alb = ld3(th->albedo);
-        if (dot(nd, N) > 0) {                          /* :242 */
-            f3 child = trace_rec(c, hit, nd, &dep);
-            col = mul(alb, child);
-        } else {
-            col = mk(0.f, 0.f, 0.f);
+    f3 thr = mk(1.f, 1.f, 1.f);
+    uint32_t dep = 0;
+    for (;;) {
+        wray r;
+        float o[3] = {ori.x, ori.y, ori.z}, d[3] = {dir.x, dir.y, dir.z};
+        wray_init(&r, o, d, 1e-3f);
+        hit32 h;
+        trace_f32(s->bvh, &r, 1e16f, &h);
+        dep++;                                             /* optx/optics_r.cu:31, camera_r.cu:146 */
+        if (h.prim == 0xFFFFFFFFu) {
+            /* __miss__ambient, optx/camera_r.cu:131-146 */
+            *depth = dep;
+            return mul(thr, sky_f(dir, ld3(s->ambient)));
         }
-        if (isnull3(c->dgv->normal)) c->dgv->normal = N;    /* :280-283 */
-        if (isnull3(c->dgv->albedo)) c->dgv->albedo = alb;
-    } else {
-        f3 d1 = unitv(dir);                            /* :372 */
-        float cos_t = fminf(dot(neg(d1), N), 1.f);     /* :373 */
-        float sin_t = sqrtf(1.f - cos_t * cos_t);      /* :374 */
-        f3 O = sub(hit, ld3(th->center));              /* :376-381 */
-        float ratio = 0.f > dot(dir, O) ? 1.f / th->index : th->index;   /* :382-384 */
-        int cannot = ratio * sin_t > 1.f;              /* :385 */
-        float schlick = schlick_f(cos_t, ratio);       /* :387-388 */
+        if (c->depth == dep) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :34-42 */
+        const orc_thing* th = &s->things[s->tri_inst[h.prim]];
+        const float* tri = s->tris + 9 * (size_t)h.prim;
+        f3 A = ld3(tri), B = ld3(tri + 3), C = ld3(tri + 6);
+        float u = h.u, v = h.v, w = 1.f - u - v;          /* :78-81 */
+        f3 hit = add(add(scl(w, A), scl(u, B)), scl(v, C)); /* :84 */
+        f3 N = unitv(cross(sub(B, A), sub(C, A)));         /* :88 */
+        if (dot(dir, N) > 0.f) N = neg(N);                 /* :89-90 */
         f3 nd;
-        if (cannot || schlick > orc_frand48_next(c->rng))   /* :391 */
-            nd = reflect_unit(d1, N);
-        else
-            nd = refract_unit(d1, N, ratio);           /* :394-397 */
-        f3 child = trace_rec(c, hit, nd, &dep);
-        col = mul(mk(1.f, 1.f, 1.f), child);           /* :425-428 */
+        if (th->type == ORC_DIFFUSE) {
+            nd = add(N, rnd_on_sphere(c->rng));            /* :108 */
+            f3 alb = ld3(th->albedo);
+            thr = mul(thr, alb);                           /* :135-138 */
+            c->dgv->normal = N;                            /* :143-146 */
+            c->dgv->albedo = alb;
+        } else if (th->type == ORC_REFLECT) {
+            f3 d1 = unitv(dir);                            /* :235 */
+            f3 refl = reflect_unit(d1, N);                 /* :236 */
+            nd = add(refl, scl(th->fuzz, rnd_in_sphere(c->rng)));  /* :238 */
+            f3 alb = ld3(th->albedo);
+            c->dgv->normal = N;                            /* :280-283 */
+            c->dgv->albedo = alb;
+            if (!(dot(nd, N) > 0)) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :242 */
+            thr = mul(thr, alb);
+        } else {
+            f3 d1 = unitv(dir);                            /* :372 */
+            float cos_t = fminf(dot(neg(d1), N), 1.f);     /* :373 */
+            float sin_t = sqrtf(1.f - cos_t * cos_t);      /* :374 */
+            f3 O = sub(hit, ld3(th->center));              /* :376-381 */
+            float ratio = 0.f > dot(dir, O) ? 1.f / th->index : th->index;   /* :382-384 */
+            int cannot = ratio * sin_t > 1.f;              /* :385 */
+            float schlick = schlick_f(cos_t, ratio);       /* :387-388 */
+            if (cannot || schlick > orc_frand48_next(c->rng))   /* :391 */
+                nd = reflect_unit(d1, N);
+            else
+                nd = refract_unit(d1, N, ratio);           /* :394-397 */
+            /* :425-428 multiply by (1,1,1): exact, left out */
+        }
+        ori = hit;
+        dir = nd;
     }
-    *depth = dep;
-    return col;
 }
 
 typedef struct {
@@ -854,7 +858,7 @@ static void* render_worker(void* arg) {
                 dgv_t dgv = {mk(0, 0, 0), mk(0, 0, 0)};
                 tctx c = {j->s, j->depth, &rng, &dgv};
                 uint32_t depth = 0;
-                f3 col = trace_rec(&c, ld3(o), ld3(d), &depth);
+                f3 col = trace_path(&c, ld3(o), ld3(d), &depth);
                 color = add(color, col);
                 sq = add(sq, mul(col, col));
                 rpp = rpp + depth;
@@ -898,7 +902,7 @@ uint64_t orc_render(const orc_scene* s, const orc_camera* cam, uint32_t w, uint3
 
 /* ========================================================================= */
 /* Frozen-structure evaluation (finite-difference checker for material      */
-/* gradients).  A sample path is traced exactly as trace_rec does it (same   */
+/* gradients).  A sample path is traced exactly as trace_path does it (same  */
 /* random draws, same decisions) with the scene's own materials, its        */
 /* vertices are recorded, and its radiance is then evaluated in FP64 along   */
 /* the same vertices with other material parameters: directions follow the  */

@@ -19,6 +19,8 @@
 
 
 
+
+
 namespace rtx {
 
 // slab test pad on the far distance: 1 + 2*gamma(3) (Ize 2013), conservative
