@@ -156,7 +156,7 @@ rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, ui
 /* Launch tuning of a context (no reference counterpart).  shade_batch: in
  * the render kernel a wave traverses until at least this many of its lanes
  * hold a final closest hit, shades those and resumes the others (1..64; 64 =
- * every lane of the wave finishes first; default 24).  inner_frac: a
+ * every lane of the wave finishes first; default 32).  inner_frac: a
  * traversal round's node phase ends when fewer than inner_frac/64 of the
  * traversing lanes can still step through inner nodes (each lane parks one
  * leaf and goes on); the parked leaves are then tested (1..64; 1 = every lane
