@@ -210,6 +210,62 @@ def diff_leg(reps):
     return rec
 
 
+def interactive_leg(frames=120):
+    """Repeated launches as the interactive viewer makes them
+    (optx/simpleui.cxx:207-245 with optx/simplesm.cxx:978-985): one frame =
+    a camera update, Launcher::ignite (synchronous) at 1 spp and pp_sRGB,
+    1280x720, depth 16, on the level-6 scene; the camera orbits by 0.5 degree
+    per frame.  Reports frames/s, the wall time per frame and the kernel
+    time per frame (HIP events), whose difference is the per-launch overhead."""
+    import math
+    import rtxplay_amd as rtx
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene
+
+    w, h = 1280, 720
+    sc = RTOWScene(seed=0, half=11, level=6)
+    sc.load()
+    lp = rtx.LpGeneral()
+    lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, 1, 16, sc.build()
+    cam = rtx.Camera()
+
+    def aim(i):
+        a = math.atan2(3.0, 13.0) + math.radians(0.5 * i)
+        r = math.hypot(13.0, 3.0)
+        cam.set((r * math.cos(a), 2.0, r * math.sin(a)), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0)
+        lp.camera = cam.c
+
+    la = rtx.Launcher(sc, lp)
+    img = rtx.DeviceBuffer(4 * w * h)
+    stream = Stream()
+    aim(0)
+    la.ignite(stream=stream)
+    rtx.pp_sRGB(lp.rawRGB, img.ptr, w, h, stream)
+    stream.synchronize()
+    ev = [(Event(), Event()) for _ in range(frames)]
+    t0 = time.perf_counter()
+    for i in range(frames):
+        aim(i + 1)
+        ev[i][0].record(stream)
+        la.ignite(stream=stream, sync=False)
+        ev[i][1].record(stream)
+        rtx.pp_sRGB(lp.rawRGB, img.ptr, w, h, stream)
+        stream.synchronize()        # Launcher::ignite is synchronous (optx/launcher.cxx:70)
+    wall = (time.perf_counter() - t0) / frames
+    kern = sum(e0.elapsed_ms(e1) for e0, e1 in ev) / frames
+    rays = int(la.rays_per_pixel().sum(dtype=np.uint64))
+    log("interactive: %.2f ms/frame wall, %.2f ms kernel, %.0f frames/s" % (wall * 1e3, kern, 1.0 / wall))
+    rec = {"workload": "rtow-final-scene level-6 spheres, 1280x720, 1 spp, depth 16, orbiting camera, "
+                       "synchronous launch + sRGB per frame", "frames": frames,
+           "frames_per_s": round(1.0 / wall, 1), "ms_per_frame": round(wall * 1e3, 3),
+           "kernel_ms_per_frame": round(kern, 3), "overhead_ms_per_frame": round(wall * 1e3 - kern, 3),
+           "rays_per_frame": rays}
+    img.free()
+    la.free()
+    sc.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,8 +276,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
-    ap.add_argument("--legs", default="frame,batch,diff",
-                    help="comma list of: frame (the headline render), batch (64 frames), diff (adjoint)")
+    ap.add_argument("--legs", default="frame,batch,diff,interactive",
+                    help="comma list of: frame (the headline render), batch (64 frames), diff (adjoint), "
+                         "interactive (repeated 1-spp launches)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -260,6 +317,7 @@ def main():
         batched = batched_leg(args.batch_reps, args.retune, log)
         log("batched speedup %.2fx" % batched["speedup"])
     diff = diff_leg(max(3, args.batch_reps)) if "diff" in legs else None
+    inter = interactive_leg() if "interactive" in legs else None
 
     cpu = None
     if not args.no_cpu and fr is not None:
@@ -282,6 +340,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
         "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
+        "interactive": inter,
     }
     print(json.dumps(out), flush=True)
 

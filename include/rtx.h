@@ -110,6 +110,26 @@ rtx_status rtx_camera_set(rtx_camera* cam, rtx_float3 eye, rtx_float3 pat, rtx_f
 rtx_status rtx_sphere_mesh(float radius, uint32_t ndiv, rtx_float3* vces, uint32_t vcap,
                            uint32_t* nv, rtx_uint3* ices, uint32_t icap, uint32_t* ni);
 
+/* Object(wavefront), optx/object.cxx:23-93 (host): reads a Wavefront OBJ file
+ * into one indexed triangle mesh per shape ("o"/"g"), each shape's vertices
+ * in first-use order and its indices renumbered into them (the reference's
+ * index-bound and renumbering defects are not reproduced).  Faces of n > 3
+ * corners become the fan (0, k, k+1).  rtx_object_mesh returns pointers into
+ * the object, valid until rtx_object_free. */
+typedef struct rtx_object rtx_object;
+rtx_status rtx_object_load(const char* wavefront, rtx_object** obj);
+rtx_status rtx_object_size(const rtx_object* obj, uint32_t* n);
+rtx_status rtx_object_mesh(const rtx_object* obj, uint32_t m, const rtx_float3** vces, uint32_t* nv,
+                           const rtx_uint3** ices, uint32_t* ni);
+rtx_status rtx_object_free(rtx_object* obj);
+/* Writes meshes as one OBJ shape named `name`: all vertices, then all faces
+ * (1-based, offset per mesh), with vertex and triangle count comments, as
+ * the sphere and object writers of optx/sphere.cxx:108-140 and
+ * optx/object.cxx:96-122 print them (coordinates with 9 significant digits,
+ * so a float reads back exactly). */
+rtx_status rtx_object_write(const char* path, const char* name, const rtx_float3* const* vces, const uint32_t* nv,
+                            const rtx_uint3* const* ices, const uint32_t* ni, uint32_t nmesh);
+
 /* Scene::add(Object&), optx/scene.cxx:38-167: uploads one mesh (the caller
  * concatenates an Object's submeshes) and returns its id. */
 rtx_status rtx_mesh_add(rtx_ctx* ctx, const rtx_float3* vces, uint32_t nv,

@@ -58,6 +58,22 @@ class Sphere {
 class Object {
    public:
     Object() {}
+    // Object(wavefront), optx/object.cxx:23-93: one submesh per OBJ shape
+    explicit Object(const std::string& wavefront) {
+        rtx_object* o = nullptr;
+        check(rtx_object_load(wavefront.c_str(), &o), "rtx_object_load");
+        unsigned int n = 0;
+        rtx_object_size(o, &n);
+        for (unsigned int m = 0; m < n; m++) {
+            const float3* v;
+            const uint3* i;
+            unsigned int nv, ni;
+            rtx_object_mesh(o, m, &v, &nv, &i, &ni);
+            vces_.emplace_back(v, v + nv);
+            ices_.emplace_back(i, i + ni);
+        }
+        rtx_object_free(o);
+    }
     void push(const std::vector<float3>& v, const std::vector<uint3>& i) {
         vces_.push_back(v);
         ices_.push_back(i);

@@ -6,5 +6,5 @@ include/rtx.h).  This package is its host-side mirror of the reference API
 """
 from ._capi import RtxError, lib, LIB_PATH  # noqa: F401
 from .api import (Batch, Camera, DeviceBuffer, Launcher, LpGeneral, Object, Scene, Sphere, Thing,  # noqa: F401
-                  TYPE_DIFFUSE, TYPE_REFLECT, TYPE_REFRACT, device_count, pp_none, pp_sRGB, rng_draws,
+                  TYPE_DIFFUSE, TYPE_REFLECT, TYPE_REFRACT, device_count, pp_none, pp_sRGB, read_obj, rng_draws, write_obj,
                   synchronize)

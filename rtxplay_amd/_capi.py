@@ -121,6 +121,12 @@ SIGNATURES = {
     "rtx_scene_set_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_scene_get_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_launch_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rtx_object_load": (c_int, [c_char_p, POINTER(c_void_p)]),
+    "rtx_object_size": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "rtx_object_mesh": (c_int, [c_void_p, c_uint32, POINTER(c_void_p), POINTER(c_uint32), POINTER(c_void_p),
+                                POINTER(c_uint32)]),
+    "rtx_object_free": (c_int, [c_void_p]),
+    "rtx_object_write": (c_int, [c_char_p, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
     "rtx_batch_create": (c_int, [c_void_p, c_void_p, c_uint32, c_int, POINTER(c_void_p)]),
     "rtx_batch_launch": (c_int, [c_void_p, c_void_p]),
     "rtx_batch_destroy": (c_int, [c_void_p]),

@@ -192,9 +192,13 @@ class Sphere:
 
 
 class Object:
-    """A list of submeshes (optx/object.h:20-33); Scene.add concatenates them."""
+    """A list of submeshes (optx/object.h:20-33); Scene.add concatenates them.
+    Object(meshes) takes (vertices, indices) pairs; Object(path) reads a
+    Wavefront OBJ file (Object(wavefront), optx/object.cxx:23-93)."""
 
     def __init__(self, meshes):
+        if isinstance(meshes, (str, bytes)):
+            meshes = read_obj(meshes)
         self.meshes = list(meshes)
 
     def __len__(self):
@@ -483,6 +487,37 @@ class Batch:
             pass
 
 
+def read_obj(path):
+    """[(vertices float32[nv,3], indices uint32[ni,3]), ...], one per OBJ shape."""
+    o = c_void_p()
+    call("rtx_object_load", path.encode() if isinstance(path, str) else path, byref(o))
+    try:
+        n = c_uint32(0)
+        call("rtx_object_size", o, byref(n))
+        out = []
+        for m in range(n.value):
+            pv, pi, nv, ni = c_void_p(), c_void_p(), c_uint32(0), c_uint32(0)
+            call("rtx_object_mesh", o, m, byref(pv), byref(nv), byref(pi), byref(ni))
+            v = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(c_float)), (nv.value * 3,)).reshape(-1, 3).copy()
+            i = np.ctypeslib.as_array(ctypes.cast(pi, ctypes.POINTER(c_uint32)), (ni.value * 3,)).reshape(-1, 3).copy()
+            out.append((v, i))
+        return out
+    finally:
+        call("rtx_object_free", o)
+
+
+def write_obj(path, meshes, name="object"):
+    """Writes (vertices, indices) meshes as one OBJ shape (optx/sphere.cxx:108-140)."""
+    vs = [np.ascontiguousarray(v, np.float32).reshape(-1, 3) for v, _ in meshes]
+    is_ = [np.ascontiguousarray(i, np.uint32).reshape(-1, 3) for _, i in meshes]
+    n = len(vs)
+    pv = (c_void_p * max(n, 1))(*[x.ctypes.data for x in vs])
+    pi = (c_void_p * max(n, 1))(*[x.ctypes.data for x in is_])
+    nv = (c_uint32 * max(n, 1))(*[len(x) for x in vs])
+    ni = (c_uint32 * max(n, 1))(*[len(x) for x in is_])
+    call("rtx_object_write", path.encode() if isinstance(path, str) else path, name.encode(), pv, nv, pi, ni, n)
+
+
 def pp_sRGB(src, dst, w, h, stream=None):
     """optx/postproc.cu:56-61 (device pointers)"""
     call("rtx_pp_srgb", c_void_p(src), c_void_p(dst), int(w), int(h), _stream_ptr(stream))
@@ -514,5 +549,6 @@ def device_count():
 
 
 __all__ = ["Camera", "Thing", "Sphere", "Object", "Scene", "LpGeneral", "Launcher", "Batch", "DeviceBuffer", "pp_sRGB",
+           "read_obj", "write_obj",
            "pp_none", "rng_draws", "device_count", "synchronize", "TYPE_DIFFUSE", "TYPE_REFLECT", "TYPE_REFRACT",
            "RtxError", "lib", "HitC", "RayC", "OpticsC"]

@@ -23,7 +23,7 @@ APP = os.path.join(PKG, "rtwo")
 ARCH = os.environ.get("RTX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["bvh_build.hip", "render.hip", "query.hip"]
-CXX_SOURCES = ["capi.cpp"]
+CXX_SOURCES = ["capi.cpp", "objio.cpp"]
 
 
 def hipcc():

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "bvh_build.h"
+#include "host_util.h"
 #include "kernels.h"
 #include "rtx.h"
 #include "rtx_internal.h"
@@ -31,6 +32,12 @@ rtx_status fail(rtx_status st, const std::string& msg) {
     g_err = msg;
     return st;
 }
+
+}  // namespace
+
+rtx_status rtx::set_error(rtx_status st, const std::string& msg) { return fail(st, msg); }
+
+namespace {
 
 rtx_status hip_fail(hipError_t e, const char* what) {
     return fail(e == hipErrorOutOfMemory ? RTX_ERROR_OUT_OF_MEMORY : RTX_ERROR_HIP,
