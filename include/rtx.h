@@ -149,22 +149,46 @@ rtx_status rtx_thing_get(rtx_ctx* ctx, uint32_t thing_id, float transform[12]);
 rtx_status rtx_thing_at(rtx_ctx* ctx, uint32_t thing_id, rtx_thing* thing);
 rtx_status rtx_thing_count(rtx_ctx* ctx, uint32_t* n);
 
-/* Scene::build, optx/scene.cxx:227-273: flattens every instance to world
- * space on the device and builds the acceleration structure there (binned
- * SAH BVH).  Synchronous.  Returns the traversable handle. */
+/* Layout of the acceleration structure the next rtx_scene_build makes.  The
+ * reference always builds an IAS over one GAS per mesh (optx/scene.cxx:38-273);
+ * RTX_BUILD_TWO_LEVEL is that design: one BVH per mesh over its object-space
+ * triangles, one over the instances' world boxes, rays transformed into an
+ * instance's object space on entry, and rtx_scene_update a refit of the top
+ * level.  RTX_BUILD_FLAT (the default) transforms every instance's triangles
+ * to world space and builds one BVH over all of them: more memory for
+ * instanced meshes, fewer node visits per ray, and rtx_scene_update a
+ * rebuild.  Both give the same hits up to float rounding. */
+enum { RTX_BUILD_FLAT = 0, RTX_BUILD_TWO_LEVEL = 1 };
+rtx_status rtx_scene_set_build_mode(rtx_ctx* ctx, int mode);
+rtx_status rtx_scene_get_build_mode(rtx_ctx* ctx, int* mode);
+
+/* Scene::build, optx/scene.cxx:227-273: builds the acceleration structure on
+ * the device (binned SAH BVH, layout by the build mode).  Synchronous.
+ * Returns the traversable handle. */
 rtx_status rtx_scene_build(rtx_ctx* ctx, void* stream, uint64_t* is_handle);
 /* Scene::update, optx/scene.cxx:275-294: brings the acceleration structure up
- * to date with rtx_thing_set changes (rebuild). Synchronous. */
+ * to date with rtx_thing_set changes and keeps the handle.  A two-level scene
+ * whose instance count is unchanged refits its top level in place (new
+ * instance boxes, same topology, as OPTIX_BUILD_OPERATION_UPDATE); otherwise
+ * the scene is rebuilt.  Material records return to the things' values.
+ * Synchronous. */
 rtx_status rtx_scene_update(rtx_ctx* ctx, uint64_t is_handle, void* stream);
 
-/* Statistics of a built scene (for reports and tests). */
+/* Statistics of a built scene (for reports and tests).  num_prims counts
+ * the triangles of all instances; for a two-level scene the node and leaf
+ * counts add up both levels, max_depth is top + deepest bottom, sah_cost
+ * is the top level's. */
 typedef struct {
     uint32_t num_prims, num_nodes, num_leaves, max_depth, max_leaf;
     float    sah_cost;          /* SAH cost, C_trav = 1, C_isect = 1 */
     float    build_ms;          /* device time of the last build */
     uint64_t device_bytes;      /* nodes + leaf-ordered triangles + instance data */
     uint32_t wide_nodes;        /* 4-wide nodes the traversal uses (collapsed from the binary tree) */
-    uint32_t wide_depth;        /* levels of the 4-wide tree */
+    uint32_t wide_depth;        /* levels of the 4-wide tree (two-level: top + deepest bottom) */
+    uint32_t two_level;         /* 1: built with RTX_BUILD_TWO_LEVEL */
+    uint32_t unique_prims;      /* triangles stored (two-level: of the meshes in use, once each) */
+    uint32_t tlas_nodes;        /* two-level: 4-wide nodes of the top level */
+    uint32_t num_blas;          /* two-level: bottom-level trees (meshes in use) */
 } rtx_scene_stats;
 rtx_status rtx_scene_stats_get(rtx_ctx* ctx, uint64_t is_handle, rtx_scene_stats* st);
 

@@ -88,7 +88,8 @@ class SceneStatsC(Structure):
     _fields_ = [("num_prims", c_uint32), ("num_nodes", c_uint32), ("num_leaves", c_uint32),
                 ("max_depth", c_uint32), ("max_leaf", c_uint32), ("sah_cost", c_float),
                 ("build_ms", c_float), ("device_bytes", c_uint64), ("wide_nodes", c_uint32),
-                ("wide_depth", c_uint32)]
+                ("wide_depth", c_uint32), ("two_level", c_uint32), ("unique_prims", c_uint32),
+                ("tlas_nodes", c_uint32), ("num_blas", c_uint32)]
 
 
 class TuningC(Structure):
@@ -110,6 +111,8 @@ SIGNATURES = {
     "rtx_thing_get": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
     "rtx_thing_at": (c_int, [c_void_p, c_uint32, POINTER(ThingC)]),
     "rtx_thing_count": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "rtx_scene_set_build_mode": (c_int, [c_void_p, c_int]),
+    "rtx_scene_get_build_mode": (c_int, [c_void_p, POINTER(c_int)]),
     "rtx_scene_build": (c_int, [c_void_p, c_void_p, POINTER(c_uint64)]),
     "rtx_scene_update": (c_int, [c_void_p, c_uint64, c_void_p]),
     "rtx_scene_stats_get": (c_int, [c_void_p, c_uint64, POINTER(SceneStatsC)]),

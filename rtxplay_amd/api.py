@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 
 from ._capi import (CameraC, Float3, HitC, LpGeneralC, OpticsC, RayC, RtxError, SceneStatsC, ThingC, TuningC, byref,
-                    c_float, c_uint32, c_uint64, c_void_p, call, lib)
+                    c_float, c_int, c_uint32, c_uint64, c_void_p, call, lib)
 
 TYPE_DIFFUSE, TYPE_REFLECT, TYPE_REFRACT, TYPE_NUM = 0, 1, 2, 3
 
@@ -286,6 +286,19 @@ class Scene:
         call("rtx_thing_count", self.ctx, byref(n))
         return n.value
 
+    FLAT, TWO_LEVEL = 0, 1
+
+    def build_mode(self, mode=None):
+        """Reads (and with an argument sets) the layout of the next build:
+        Scene.FLAT (one BVH over world-space triangles, the default) or
+        Scene.TWO_LEVEL (per-mesh BVHs under an instance BVH, as the
+        reference's IAS over GASes, optx/scene.cxx:38-273)."""
+        if mode is not None:
+            call("rtx_scene_set_build_mode", self.ctx, int(mode))
+        m = c_int(0)
+        call("rtx_scene_get_build_mode", self.ctx, byref(m))
+        return m.value
+
     def build(self, stream=None):
         """Device build; returns the traversable handle (optx/scene.cxx:227-273)."""
         h = c_uint64(0)
@@ -294,7 +307,8 @@ class Scene:
         return h.value
 
     def update(self, handle, stream=None):
-        """Refreshes the structure after set(); the handle stays valid (optx/scene.cxx:275-294)."""
+        """Refreshes the structure after set(); the handle stays valid
+        (optx/scene.cxx:275-294).  Two-level scenes refit their top level."""
         call("rtx_scene_update", self.ctx, c_uint64(handle), _stream_ptr(stream))
         return handle
 

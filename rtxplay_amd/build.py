@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "librtx.so")
 APP = os.path.join(PKG, "rtwo")
 ARCH = os.environ.get("RTX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["bvh_build.hip", "render.hip", "query.hip"]
+HIP_SOURCES = ["bvh_build.hip", "tlas.hip", "render.hip", "query.hip"]
 CXX_SOURCES = ["capi.cpp", "objio.cpp"]
 
 

@@ -4,7 +4,10 @@
 // with: instance flattening to world space, a top-down binned SAH BVH built
 // level by level on the device (Wald 2007, "On fast construction of SAH-based
 // bounding volume hierarchies"), triangle reordering into leaf order, and a
-// conversion to the 64-byte two-child node layout the traversal reads.
+// collapse into the 128-byte 4-wide node layout the traversal reads.  The
+// same builder makes the levels of a two-level scene: one tree per mesh over
+// its object-space triangles and one over the instances' world boxes (one
+// instance per leaf); tlas.hip assembles and refits them.
 //
 // Determinism: every level is processed in task order, child nodes and tasks
 // are placed by an exclusive scan (no atomics decide positions) and the
@@ -183,7 +186,7 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_
 __global__ __launch_bounds__(kBuildBlock) void k_level(
         const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
         const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
-        BNode* __restrict__ bnodes, SplitOut* __restrict__ out) {
+        BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
     __shared__ uint32_t s_cnt[3][kBins];
     __shared__ uint32_t s_b[3][kBins][12];   // lo.xyz, hi.xyz, clo.xyz, chi.xyz (ordered ints)
     __shared__ float s_rarea[3][kBins];
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
         const float split_cost = kCostTrav * A + kCostIsect * best;
         int kind;   // 0 leaf, 1 SAH split, 2 median split
         if (n <= 1) kind = 0;
-        else if (n <= kMaxLeaf && (axis < 0 || leaf_cost <= split_cost)) kind = 0;
+        else if (n <= max_leaf && (axis < 0 || leaf_cost <= split_cost)) kind = 0;
         else if (axis < 0 || task.depth >= (uint32_t)(kMaxTreeDepth - 28)) kind = 2;
         else kind = 1;
         s_kind = kind;
@@ -606,8 +609,9 @@ hipError_t flatten_scene(const InstBuild* d_insts, uint32_t ninst, const MeshRef
 }
 
 hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float4* d_phi, uint32_t N,
-                     hipStream_t s, BuiltBVH* outp, std::string* err) {
+                     hipStream_t s, BuiltBVH* outp, std::string* err, const BvhOptions& opt) {
     BuiltBVH out;
+    const uint32_t max_leaf = opt.max_leaf < 1 ? 1u : (opt.max_leaf > kMaxLeaf ? kMaxLeaf : opt.max_leaf);
     if (N == 0) {
         out.root = kEmptyRoot;
         *outp = out;
@@ -639,7 +643,7 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     BuildTask* nxt = tasks_b.p;
     while (ntasks > 0) {
         hipLaunchKernelGGL(k_level, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, tmp.p,
-                           bnodes.p, so.p);
+                           bnodes.p, so.p, max_leaf);
         hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, ntasks, rank.p, total.p);
         hipLaunchKernelGGL(k_emit, dim3(blocks_for(ntasks, 256)), dim3(256), 0, s, cur, so.p, rank.p, ntasks,
                            nnodes, bnodes.p, nxt, st.p);
@@ -666,10 +670,12 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     RTX_TRY(hipStreamSynchronize(s));
 
     DevBuf<float4> nodes, tris;
-    RTX_TRY(tris.alloc(3 * ((size_t)N + 1)));
-    hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for((uint64_t)N + 1, 256)), dim3(256), 0, s, d_tri_world, idx.p, N,
-                       tris.p);
-    RTX_TRY(hipGetLastError());
+    if (opt.gather) {
+        RTX_TRY(tris.alloc(3 * ((size_t)N + 1)));
+        hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for((uint64_t)N + 1, 256)), dim3(256), 0, s, d_tri_world,
+                           idx.p, N, tris.p);
+        RTX_TRY(hipGetLastError());
+    }
 
     // 4-wide nodes: at most one per binary inner node
     uint32_t nwide = 0, wide_depth = 0;
@@ -689,6 +695,7 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         uint32_t* cur = fa.p;
         uint32_t* nxt = fb.p;
         while (n > 0) {
+            out.level_begin.push_back(nwide);
             hipLaunchKernelGGL(k_collapse4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cur, n, cid.p,
                                cnt.p);
             hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, cnt.p, n, off.p, total.p);
@@ -710,12 +717,16 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         RTX_TRY(nodes.alloc(kWideNodeF4));
     }
     RTX_TRY(hipStreamSynchronize(s));
+    out.level_begin.push_back(nwide);
 
     const uint32_t root_cnt = __builtin_bit_cast(uint32_t, root.hi.w);
     const float rx = root.hi.x - root.lo.x, ry = root.hi.y - root.lo.y, rz = root.hi.z - root.lo.z;
     const double root_area = 2.0 * ((double)rx * ry + (double)ry * rz + (double)rz * rx);
     out.nodes = nodes.release();
     out.tris = tris.release();
+    if (!opt.gather) out.order = idx.release();
+    out.root_lo[0] = root.lo.x; out.root_lo[1] = root.lo.y; out.root_lo[2] = root.lo.z;
+    out.root_hi[0] = root.hi.x; out.root_hi[1] = root.hi.y; out.root_hi[2] = root.hi.z;
     out.root = root_cnt ? leaf_ref(__builtin_bit_cast(uint32_t, root.lo.w), root_cnt) : 0u;
     out.num_nodes = ninner;
     out.num_wide = nwide;

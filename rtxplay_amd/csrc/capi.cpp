@@ -68,17 +68,28 @@ struct Built {
     float4* nodes = nullptr;
     float4* tris = nullptr;
     float4* insts = nullptr;
-    float4* tri_world = nullptr;       // original-order world triangles (3 float4 each)
+    float4* tri_world = nullptr;       // flat: original-order world triangles (3 float4 each)
     int device = 0;
     uint32_t stack_need = 1;           // traversal stack entries per lane this scene can need
     rtx_scene_stats stats{};
+    // two-level scenes
+    bool two_level = false;
+    float4* xinst = nullptr;           // instance records (kInstF4 float4 each, + the null instance)
+    float4* ibox = nullptr;            // instance world boxes (lo, hi), + the null instance
+    std::vector<uint32_t> tlas_levels; // first top-level node of each level, then the count
+    std::vector<uint32_t> blas_root;   // per mesh: bottom-level root in the scene arrays
+    std::vector<float> blas_box;       // per mesh: object-space bounds lo.xyz, hi.xyz
+    uint32_t null_root = 0;            // a one-triangle leaf of a degenerate triangle
+    std::vector<rtx::InstBuild> ib;    // transforms and first global triangle per instance at the last build/update
     void release() {
         if (d_hdr) (void)hipFree(d_hdr);
         if (nodes) (void)hipFree(nodes);
         if (tris) (void)hipFree(tris);
         if (insts) (void)hipFree(insts);
         if (tri_world) (void)hipFree(tri_world);
-        d_hdr = nullptr; nodes = tris = insts = tri_world = nullptr;
+        if (xinst) (void)hipFree(xinst);
+        if (ibox) (void)hipFree(ibox);
+        d_hdr = nullptr; nodes = tris = insts = tri_world = xinst = ibox = nullptr;
     }
 };
 
@@ -111,6 +122,7 @@ struct rtx_ctx {
     Built* built = nullptr;
     rtx_float3 ambient{.5f, .7f, 1.f};
     rtx::LaunchScratch render_ls, trace_ls;
+    int build_mode = RTX_BUILD_FLAT;
     uint32_t* tape = nullptr;       // adjoint path records
     uint32_t tape_depth = 0;        // depth the tape was sized for
 };
@@ -399,13 +411,15 @@ rtx_status rtx_set_miss_color(rtx_ctx* c, rtx_float3 ambient) {
     return RTX_SUCCESS;
 }
 
-// Flattens every instance and builds the BVH into a new Built (not registered).
-static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
-    HIP_OK(hipSetDevice(c->device));
+// Per instance: the flatten record (transform, mesh, first global triangle)
+// and the material record of the scene (2 float4); N = triangles of all
+// instances.
+static void instance_records(const rtx_ctx* c, std::vector<rtx::InstBuild>& ib, std::vector<float4>& irec,
+                             uint64_t& N) {
     const uint32_t ninst = (uint32_t)c->things.size();
-    std::vector<rtx::InstBuild> ib(ninst ? ninst : 1);
-    std::vector<float4> irec(2 * (size_t)(ninst ? ninst : 1));
-    uint64_t N = 0;
+    ib.assign(ninst ? ninst : 1, rtx::InstBuild{});
+    irec.assign(2 * (size_t)(ninst ? ninst : 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    N = 0;
     for (uint32_t i = 0; i < ninst; i++) {
         const ThingRec& t = c->things[i];
         std::memcpy(ib[i].xf, t.xf, sizeof(t.xf));
@@ -425,6 +439,16 @@ static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
         irec[2 * i] = make_float4(alb.x, alb.y, alb.z, e);
         irec[2 * i + 1] = make_float4(t.xf[3], t.xf[7], t.xf[11], tyf);   // center: optx/optics_r.cu:376-380
     }
+}
+
+// Flattens every instance and builds the BVH into a new Built (not registered).
+static rtx_status build_scene_flat(rtx_ctx* c, hipStream_t s, Built** out) {
+    HIP_OK(hipSetDevice(c->device));
+    const uint32_t ninst = (uint32_t)c->things.size();
+    std::vector<rtx::InstBuild> ib;
+    std::vector<float4> irec;
+    uint64_t N = 0;
+    instance_records(c, ib, irec, N);
     if (N > rtx::kMaxPrims) return fail(RTX_ERROR_LIMIT, "rtx_scene_build: more than 2^27-1 triangles");
 
     hipEvent_t e0, e1;
@@ -461,6 +485,7 @@ static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
     b->hdr.nodes = bvh.nodes;
     b->hdr.tris = bvh.tris;
     b->hdr.insts = b->insts;
+    b->hdr.xinst = nullptr;
     b->hdr.root = bvh.root;
     b->hdr.num_prims = (uint32_t)N;
     b->hdr.num_nodes = bvh.num_nodes;
@@ -493,9 +518,381 @@ static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
     b->stats.device_bytes = sizeof(float4) * ((uint64_t)rtx::kWideNodeF4 * bvh.num_wide + 3ull * N + 2ull * ninst);
     b->stats.wide_nodes = bvh.num_wide;
     b->stats.wide_depth = bvh.wide_depth;
+    b->stats.unique_prims = (uint32_t)N;
+    b->ib = ib;
     // a step pushes at most 3 entries per level of the 4-wide tree
     b->stack_need = 3u * (bvh.wide_depth > 0 ? bvh.wide_depth : 1u);
     *out = b;
+    return RTX_SUCCESS;
+}
+
+// ---- two-level scenes -------------------------------------------------------
+namespace {
+
+// owned results of build_bvh, freed at scope exit
+struct BvhSet {
+    std::vector<rtx::BuiltBVH> v;
+    ~BvhSet() {
+        for (auto& x : v) {
+            if (x.nodes) (void)hipFree(x.nodes);
+            if (x.tris) (void)hipFree(x.tris);
+            if (x.order) (void)hipFree(x.order);
+        }
+    }
+};
+
+// One instance of a two-level scene: its record (world-to-object rows,
+// object-to-world rows, (root, first global triangle, mesh, 0)) and its world
+// box from the 8 corners of the mesh's object-space box, widened by 2^-20 of
+// its largest coordinate so that the float rounding of the corners cannot cut
+// off a hit.  A transform without an inverse makes the instance unhittable:
+// the null root and a point box at its translation.
+void tl_instance(const rtx::InstBuild& in, uint32_t root, uint32_t null_root, const float* box, float4* rec,
+                 float4* wb) {
+    double m[3][3], t[3];
+    for (int r = 0; r < 3; r++) {
+        for (int k = 0; k < 3; k++) m[r][k] = in.xf[4 * r + k];
+        t[r] = in.xf[4 * r + 3];
+    }
+    const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    double inv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    bool ok = std::isfinite(det) && det != 0.0;
+    if (ok) {
+        const double id = 1.0 / det;
+        inv[0][0] = c00 * id;
+        inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+        inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+        inv[1][0] = c01 * id;
+        inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+        inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+        inv[2][0] = c02 * id;
+        inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+        inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    }
+    float w2o[12];
+    for (int r = 0; r < 3 && ok; r++) {
+        double tr = 0.0;
+        for (int k = 0; k < 3; k++) {
+            w2o[4 * r + k] = (float)inv[r][k];
+            tr -= inv[r][k] * t[k];
+        }
+        w2o[4 * r + 3] = (float)tr;
+        for (int k = 0; k < 4; k++) ok = ok && std::isfinite(w2o[4 * r + k]);
+    }
+    if (!ok) std::memcpy(w2o, kIdentity, sizeof(w2o));
+    for (int r = 0; r < 3; r++) {
+        rec[r] = make_float4(w2o[4 * r], w2o[4 * r + 1], w2o[4 * r + 2], w2o[4 * r + 3]);
+        rec[3 + r] = make_float4(in.xf[4 * r], in.xf[4 * r + 1], in.xf[4 * r + 2], in.xf[4 * r + 3]);
+    }
+    uint32_t meta[4] = {ok ? root : null_root, in.tri_base, in.mesh, 0u};
+    std::memcpy(&rec[6], meta, sizeof(meta));
+    rec[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!ok) {
+        wb[0] = wb[1] = make_float4(in.xf[3], in.xf[7], in.xf[11], 0.f);
+        return;
+    }
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int cn = 0; cn < 8; cn++) {
+        const double p[3] = {box[(cn & 1) ? 3 : 0], box[(cn & 2) ? 4 : 1], box[(cn & 4) ? 5 : 2]};
+        for (int r = 0; r < 3; r++) {
+            const double w = m[r][0] * p[0] + m[r][1] * p[1] + m[r][2] * p[2] + t[r];
+            lo[r] = std::min(lo[r], w);
+            hi[r] = std::max(hi[r], w);
+        }
+    }
+    double big = 0.0;
+    for (int r = 0; r < 3; r++) big = std::max(big, std::max(std::fabs(lo[r]), std::fabs(hi[r])));
+    const double pad = big * 0x1p-20 + 1e-30;
+    wb[0] = make_float4((float)(lo[0] - pad), (float)(lo[1] - pad), (float)(lo[2] - pad), 0.f);
+    wb[1] = make_float4((float)(hi[0] + pad), (float)(hi[1] + pad), (float)(hi[2] + pad), 0.f);
+}
+
+const float4 kFarBox = make_float4(1e38f, 1e38f, 1e38f, 0.f);   // the empty-slot box of the builder
+
+// records and boxes of all instances (+ the null instance at ninst)
+void tl_records(const rtx_ctx* c, const Built* b, std::vector<float4>& xrec, std::vector<float4>& wbox) {
+    const uint32_t ninst = (uint32_t)c->things.size();
+    xrec.assign((size_t)rtx::kInstF4 * (ninst + 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    wbox.assign(2 * (size_t)(ninst + 1), kFarBox);
+    for (uint32_t i = 0; i < ninst; i++) {
+        const uint32_t m = b->ib[i].mesh;
+        float4* rec = &xrec[(size_t)rtx::kInstF4 * i];
+        if (c->meshes[m].ni == 0) {
+            // no triangles: not in the top level; a record for completeness
+            float4 wb[2];
+            tl_instance(b->ib[i], b->null_root, b->null_root, &b->blas_box[6 * (size_t)m], rec, wb);
+            continue;
+        }
+        tl_instance(b->ib[i], b->blas_root[m], b->null_root, &b->blas_box[6 * (size_t)m], rec, &wbox[2 * (size_t)i]);
+    }
+    rtx::InstBuild id{};
+    std::memcpy(id.xf, kIdentity, sizeof(id.xf));
+    float4 wb[2];
+    const float zero_box[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    tl_instance(id, b->null_root, b->null_root, zero_box, &xrec[(size_t)rtx::kInstF4 * ninst], wb);
+}
+
+}  // namespace
+
+// Builds a two-level scene into a new Built (not registered): one tree per
+// mesh in use over its object-space triangles, one over the instances.
+static rtx_status build_scene_tl(rtx_ctx* c, hipStream_t s, Built** out) {
+    HIP_OK(hipSetDevice(c->device));
+    const uint32_t ninst = (uint32_t)c->things.size();
+    if (ninst >= rtx::kLeafFirstMask) return fail(RTX_ERROR_LIMIT, "rtx_scene_build: 2^27-1 or more instances");
+    std::vector<rtx::InstBuild> ib;
+    std::vector<float4> irec;
+    uint64_t N = 0;
+    instance_records(c, ib, irec, N);
+    if (N >= 0xFFFFFFFFull) return fail(RTX_ERROR_LIMIT, "rtx_scene_build: 2^32-1 or more instanced triangles");
+    const size_t nm = c->meshes.size();
+    std::vector<char> used(nm, 0);
+    for (uint32_t i = 0; i < ninst; i++)
+        if (c->meshes[c->things[i].mesh].ni) used[c->things[i].mesh] = 1;
+
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, s));
+    Built* b = new Built();
+    b->device = c->device;
+    b->two_level = true;
+    b->ib = ib;
+    auto bail = [&](rtx_status st) { b->release(); delete b; (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; };
+    hipError_t e;
+    std::string err;
+
+    // bottom level: one tree per mesh in use
+    BvhSet bl;
+    bl.v.resize(nm);
+    b->blas_root.assign(nm, rtx::kEmptyRoot);
+    b->blas_box.assign(6 * (nm ? nm : 1), 0.f);
+    {
+        uint32_t nmax = 0;
+        for (size_t m = 0; m < nm; m++)
+            if (used[m]) nmax = std::max(nmax, c->meshes[m].ni);
+        std::vector<rtx::MeshRef> mr(nm ? nm : 1);
+        for (size_t m = 0; m < nm; m++)
+            mr[m] = rtx::MeshRef{c->meshes[m].d_v, c->meshes[m].d_i, c->meshes[m].nv, c->meshes[m].ni};
+        Tmp<rtx::InstBuild> d_one;
+        Tmp<rtx::MeshRef> d_mr;
+        Tmp<float4> d_tw, d_plo, d_phi;
+        if ((e = hipMalloc((void**)&d_one.p, sizeof(rtx::InstBuild) * nm + sizeof(rtx::InstBuild))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(insts)"));
+        if ((e = hipMalloc((void**)&d_mr.p, sizeof(rtx::MeshRef) * mr.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(meshes)"));
+        if ((e = hipMalloc((void**)&d_tw.p, sizeof(float4) * 3 * (nmax ? nmax : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(triangles)"));
+        if ((e = hipMalloc((void**)&d_plo.p, sizeof(float4) * (nmax ? nmax : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(bounds)"));
+        if ((e = hipMalloc((void**)&d_phi.p, sizeof(float4) * (nmax ? nmax : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(bounds)"));
+        if ((e = hipMemcpyAsync(d_mr.p, mr.data(), sizeof(rtx::MeshRef) * mr.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload meshes"));
+        // one identity instance per mesh (object space), uploaded at once
+        std::vector<rtx::InstBuild> one(nm ? nm : 1);
+        for (size_t m = 0; m < nm; m++) {
+            std::memcpy(one[m].xf, kIdentity, sizeof(one[m].xf));
+            one[m].mesh = (uint32_t)m;
+            one[m].tri_base = 0;
+        }
+        if ((e = hipMemcpyAsync(d_one.p, one.data(), sizeof(rtx::InstBuild) * one.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload insts"));
+        for (size_t m = 0; m < nm; m++) {
+            if (!used[m]) continue;
+            const uint32_t ni = c->meshes[m].ni;
+            if ((e = rtx::flatten_scene(d_one.p + m, 1, d_mr.p, ni, d_tw.p, d_plo.p, d_phi.p, s)) != hipSuccess) return bail(hip_fail(e, "flatten"));
+            if ((e = rtx::build_bvh(d_tw.p, d_plo.p, d_phi.p, ni, s, &bl.v[m], &err)) != hipSuccess)
+                return bail(fail(e == hipErrorOutOfMemory ? RTX_ERROR_OUT_OF_MEMORY : RTX_ERROR_HIP, "build_bvh: " + err));
+            for (int k = 0; k < 3; k++) {
+                b->blas_box[6 * m + k] = bl.v[m].root_lo[k];
+                b->blas_box[6 * m + 3 + k] = bl.v[m].root_hi[k];
+            }
+        }
+    }
+
+    // scene arrays: top-level nodes first, then each mesh's tree; each
+    // mesh's triangles with its degenerate one
+    std::vector<uint64_t> tri_base(nm, 0), node_off(nm, 0);
+    uint64_t tri_total = 0, blas_nodes = 0;
+    uint32_t blas_depth = 0, blas_wide_depth = 0, max_leaf = 0, num_nodes = 0, num_leaves = 0;
+    int first_used = -1;
+    for (size_t m = 0; m < nm; m++) {
+        if (!used[m]) continue;
+        if (first_used < 0) first_used = (int)m;
+        tri_base[m] = tri_total;
+        tri_total += (uint64_t)c->meshes[m].ni + 1;
+        node_off[m] = blas_nodes;
+        blas_nodes += bl.v[m].num_wide;
+        blas_depth = std::max(blas_depth, bl.v[m].max_depth);
+        blas_wide_depth = std::max(blas_wide_depth, bl.v[m].wide_depth);
+        max_leaf = std::max(max_leaf, bl.v[m].max_leaf);
+        num_nodes += bl.v[m].num_nodes;
+        num_leaves += bl.v[m].num_leaves;
+    }
+    if (tri_total > rtx::kLeafFirstMask || tri_total * 48u >= (1ull << 32))
+        return bail(fail(RTX_ERROR_LIMIT, "rtx_scene_build: mesh triangles above 2^27-1 or 4 GiB"));
+    b->null_root = first_used >= 0 ? rtx::leaf_ref((uint32_t)(tri_base[first_used] + c->meshes[first_used].ni), 1u)
+                                   : rtx::kEmptyRoot;
+
+    // top level over the instance boxes (instances of meshes with triangles)
+    std::vector<float4> xrec, wbox;
+    {
+        // roots are needed for the records only, boxes now
+        b->blas_root.assign(nm, b->null_root);
+        tl_records(c, b, xrec, wbox);
+    }
+    std::vector<uint32_t> tl_inst;
+    std::vector<float4> tplo, tphi;
+    for (uint32_t i = 0; i < ninst; i++) {
+        if (!c->meshes[c->things[i].mesh].ni) continue;
+        tl_inst.push_back(i);
+        tplo.push_back(wbox[2 * (size_t)i]);
+        tphi.push_back(wbox[2 * (size_t)i + 1]);
+    }
+    const uint32_t M = (uint32_t)tl_inst.size();
+    BvhSet tl;
+    tl.v.resize(1);
+    rtx::BuiltBVH& tb = tl.v[0];
+    Tmp<uint32_t> d_inst_of;
+    if (M) {
+        Tmp<float4> d_tplo, d_tphi;
+        if ((e = hipMalloc((void**)&d_tplo.p, sizeof(float4) * M)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(instance boxes)"));
+        if ((e = hipMalloc((void**)&d_tphi.p, sizeof(float4) * M)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(instance boxes)"));
+        if ((e = hipMalloc((void**)&d_inst_of.p, sizeof(uint32_t) * M)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(instance ids)"));
+        if ((e = hipMemcpyAsync(d_tplo.p, tplo.data(), sizeof(float4) * M, hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload boxes"));
+        if ((e = hipMemcpyAsync(d_tphi.p, tphi.data(), sizeof(float4) * M, hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload boxes"));
+        if ((e = hipMemcpyAsync(d_inst_of.p, tl_inst.data(), sizeof(uint32_t) * M, hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload ids"));
+        rtx::BvhOptions opt;
+        opt.max_leaf = 1;
+        opt.gather = false;
+        if ((e = rtx::build_bvh(nullptr, d_tplo.p, d_tphi.p, M, s, &tb, &err, opt)) != hipSuccess)
+            return bail(fail(e == hipErrorOutOfMemory ? RTX_ERROR_OUT_OF_MEMORY : RTX_ERROR_HIP, "build_bvh (top level): " + err));
+    }
+    const uint64_t total_nodes = (uint64_t)tb.num_wide + blas_nodes;
+    if (total_nodes * 16u * rtx::kWideNodeF4 >= (1ull << 32))
+        return bail(fail(RTX_ERROR_LIMIT, "rtx_scene_build: node array above 4 GiB"));
+
+    if ((e = hipMalloc((void**)&b->nodes, sizeof(float4) * rtx::kWideNodeF4 * (total_nodes ? total_nodes : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(nodes)"));
+    if ((e = hipMalloc((void**)&b->tris, sizeof(float4) * 3 * (tri_total ? tri_total : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(triangles)"));
+    if (tb.num_wide) {
+        if ((e = hipMemcpyAsync(b->nodes, tb.nodes, sizeof(float4) * rtx::kWideNodeF4 * tb.num_wide, hipMemcpyDeviceToDevice, s)) != hipSuccess) return bail(hip_fail(e, "copy top level"));
+        if ((e = rtx::tlas_leaves(b->nodes, tb.num_wide, tb.order, M, d_inst_of.p, ninst, s)) != hipSuccess) return bail(hip_fail(e, "top-level leaves"));
+    }
+    for (size_t m = 0; m < nm; m++) {
+        if (!used[m]) continue;
+        const rtx::BuiltBVH& x = bl.v[m];
+        const uint32_t nb = (uint32_t)(tb.num_wide + node_off[m]), tbm = (uint32_t)tri_base[m];
+        if (x.num_wide) {
+            if ((e = hipMemcpyAsync(b->nodes + (size_t)rtx::kWideNodeF4 * nb, x.nodes, sizeof(float4) * rtx::kWideNodeF4 * x.num_wide, hipMemcpyDeviceToDevice, s)) != hipSuccess) return bail(hip_fail(e, "copy bottom level"));
+            if ((e = rtx::relocate_blas(b->nodes, x.num_wide, nb, tbm, s)) != hipSuccess) return bail(hip_fail(e, "relocate"));
+        }
+        if ((e = hipMemcpyAsync(b->tris + 3 * (size_t)tbm, x.tris, sizeof(float4) * 3 * ((size_t)c->meshes[m].ni + 1), hipMemcpyDeviceToDevice, s)) != hipSuccess) return bail(hip_fail(e, "copy triangles"));
+        b->blas_root[m] = (x.root & rtx::kLeafFlag) ? x.root + tbm : x.root + nb;
+    }
+    // records with the roots, then the top level's boxes from them
+    tl_records(c, b, xrec, wbox);
+    if ((e = hipMalloc((void**)&b->xinst, sizeof(float4) * xrec.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(instance records)"));
+    if ((e = hipMalloc((void**)&b->ibox, sizeof(float4) * wbox.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(instance boxes)"));
+    if ((e = hipMalloc((void**)&b->insts, sizeof(float4) * irec.size())) != hipSuccess) return bail(hip_fail(e, "hipMalloc(inst records)"));
+    if ((e = hipMemcpyAsync(b->xinst, xrec.data(), sizeof(float4) * xrec.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload instance records"));
+    if ((e = hipMemcpyAsync(b->ibox, wbox.data(), sizeof(float4) * wbox.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload instance boxes"));
+    if ((e = hipMemcpyAsync(b->insts, irec.data(), sizeof(float4) * irec.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload inst records"));
+    b->tlas_levels = tb.level_begin;
+    if (tb.num_wide) {
+        if ((e = rtx::refit_tlas(b->nodes, b->tlas_levels, b->ibox, rtx::kInstLeaf | ninst, s)) != hipSuccess) return bail(hip_fail(e, "refit"));
+    }
+
+    uint32_t root = rtx::kEmptyRoot;
+    if (M == 1) root = rtx::kInstLeaf | tl_inst[0];
+    else if (M > 1) root = tb.root;
+    b->hdr.nodes = b->nodes;
+    b->hdr.tris = b->tris;
+    b->hdr.insts = b->insts;
+    b->hdr.xinst = b->xinst;
+    b->hdr.root = root;
+    b->hdr.num_prims = (uint32_t)N;
+    b->hdr.num_nodes = (uint32_t)total_nodes;
+    b->hdr.num_insts = ninst;
+    if ((e = hipMalloc((void**)&b->d_hdr, sizeof(rtx::DevScene))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(handle)"));
+    if ((e = hipMemcpyAsync(b->d_hdr, &b->hdr, sizeof(rtx::DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload handle"));
+    if ((e = hipEventRecord(e1, s)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (std::max(blas_depth, tb.max_depth) > (uint32_t)rtx::kMaxTreeDepth) {
+        b->release();
+        delete b;
+        return fail(RTX_ERROR_LIMIT, "rtx_scene_build: tree deeper than the traversal stack");
+    }
+    uint64_t unique = 0;
+    for (size_t m = 0; m < nm; m++)
+        if (used[m]) unique += c->meshes[m].ni;
+    b->stats.num_prims = (uint32_t)N;
+    b->stats.num_nodes = num_nodes + tb.num_nodes;
+    b->stats.num_leaves = num_leaves + tb.num_leaves;
+    b->stats.max_depth = tb.max_depth + blas_depth;
+    b->stats.max_leaf = max_leaf;
+    b->stats.sah_cost = (float)tb.sah;
+    b->stats.build_ms = ms;
+    b->stats.device_bytes = sizeof(float4) * ((uint64_t)rtx::kWideNodeF4 * total_nodes + 3ull * tri_total + irec.size() +
+                                              xrec.size() + wbox.size());
+    b->stats.wide_nodes = (uint32_t)total_nodes;
+    b->stats.wide_depth = tb.wide_depth + blas_wide_depth;
+    b->stats.two_level = 1;
+    b->stats.unique_prims = (uint32_t)unique;
+    b->stats.tlas_nodes = tb.num_wide;
+    uint32_t nblas = 0;
+    for (size_t m = 0; m < nm; m++) nblas += used[m] ? 1u : 0u;
+    b->stats.num_blas = nblas;
+    // top-level steps, the world ray and marker, bottom-level steps
+    b->stack_need = 3u * tb.wide_depth + 7u + 3u * (blas_wide_depth > 0 ? blas_wide_depth : 1u);
+    *out = b;
+    return RTX_SUCCESS;
+}
+
+// Scene::update of a two-level scene: new instance records and boxes from
+// the current transforms, and a refit of the top level (same topology, as
+// the reference's OPTIX_BUILD_OPERATION_UPDATE, optx/scene.cxx:275-294)
+static rtx_status refit_scene_tl(rtx_ctx* c, Built* b, hipStream_t s) {
+    HIP_OK(hipSetDevice(c->device));
+    const uint32_t ninst = (uint32_t)c->things.size();
+    std::vector<float4> irec, xrec, wbox;
+    uint64_t N = 0;
+    instance_records(c, b->ib, irec, N);
+    tl_records(c, b, xrec, wbox);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->xinst, xrec.data(), sizeof(float4) * xrec.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->ibox, wbox.data(), sizeof(float4) * wbox.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->insts, irec.data(), sizeof(float4) * irec.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && b->tlas_levels.size() > 1) e = rtx::refit_tlas(b->nodes, b->tlas_levels, b->ibox, rtx::kInstLeaf | ninst, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float ms = 0.f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "rtx_scene_update (refit)");
+    b->stats.build_ms = ms;
+    return RTX_SUCCESS;
+}
+
+static rtx_status build_scene(rtx_ctx* c, hipStream_t s, Built** out) {
+    return c->build_mode == RTX_BUILD_TWO_LEVEL ? build_scene_tl(c, s, out) : build_scene_flat(c, s, out);
+}
+
+rtx_status rtx_scene_set_build_mode(rtx_ctx* c, int mode) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_set_build_mode: ctx is NULL");
+    if (mode != RTX_BUILD_FLAT && mode != RTX_BUILD_TWO_LEVEL)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_set_build_mode: unknown mode");
+    c->build_mode = mode;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_scene_get_build_mode(rtx_ctx* c, int* mode) {
+    if (!c || !mode) return fail(RTX_ERROR_INVALID_VALUE, "rtx_scene_get_build_mode: NULL argument");
+    *mode = c->build_mode;
     return RTX_SUCCESS;
 }
 
@@ -523,8 +920,11 @@ rtx_status rtx_scene_update(rtx_ctx* c, uint64_t is_handle, void* stream) {
     if (!c->built || (uint64_t)(uintptr_t)c->built->d_hdr != is_handle)
         return fail(RTX_ERROR_INVALID_HANDLE, "rtx_scene_update: handle is not this context's scene");
     hipStream_t s = (hipStream_t)stream;
+    // a two-level scene with the same instances refits its top level in place
+    if (c->built->two_level && c->built->hdr.num_insts == (uint32_t)c->things.size())
+        return refit_scene_tl(c, c->built, s);
     Built* b = nullptr;
-    rtx_status st = build_scene(c, s, &b);
+    rtx_status st = c->built->two_level ? build_scene_tl(c, s, &b) : build_scene_flat(c, s, &b);
     if (st != RTX_SUCCESS) return st;
     Built* old = c->built;
     HIP_OK(hipMemcpyAsync(old->d_hdr, &b->hdr, sizeof(rtx::DevScene), hipMemcpyHostToDevice, s));
@@ -558,7 +958,27 @@ rtx_status rtx_scene_triangles(rtx_ctx* c, uint64_t is_handle, float* tris, uint
     if (N == 0) return RTX_SUCCESS;
     HIP_OK(hipSetDevice(b->device));
     std::vector<float4> h(3 * (size_t)N);
-    HIP_OK(hipMemcpy(h.data(), b->tri_world, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    if (b->two_level) {
+        // a two-level scene keeps no world triangles: flatten them for the copy
+        const uint32_t ninst = b->hdr.num_insts;
+        std::vector<rtx::MeshRef> mr(c->meshes.size() ? c->meshes.size() : 1);
+        for (size_t m = 0; m < c->meshes.size(); m++)
+            mr[m] = rtx::MeshRef{c->meshes[m].d_v, c->meshes[m].d_i, c->meshes[m].nv, c->meshes[m].ni};
+        Tmp<rtx::InstBuild> d_ib;
+        Tmp<rtx::MeshRef> d_mr;
+        Tmp<float4> d_tw, d_plo, d_phi;
+        HIP_OK(hipMalloc((void**)&d_ib.p, sizeof(rtx::InstBuild) * b->ib.size()));
+        HIP_OK(hipMalloc((void**)&d_mr.p, sizeof(rtx::MeshRef) * mr.size()));
+        HIP_OK(hipMalloc((void**)&d_tw.p, sizeof(float4) * h.size()));
+        HIP_OK(hipMalloc((void**)&d_plo.p, sizeof(float4) * N));
+        HIP_OK(hipMalloc((void**)&d_phi.p, sizeof(float4) * N));
+        HIP_OK(hipMemcpy(d_ib.p, b->ib.data(), sizeof(rtx::InstBuild) * b->ib.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d_mr.p, mr.data(), sizeof(rtx::MeshRef) * mr.size(), hipMemcpyHostToDevice));
+        HIP_OK(rtx::flatten_scene(d_ib.p, ninst, d_mr.p, N, d_tw.p, d_plo.p, d_phi.p, nullptr));
+        HIP_OK(hipMemcpy(h.data(), d_tw.p, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    } else {
+        HIP_OK(hipMemcpy(h.data(), b->tri_world, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    }
     for (size_t p = 0; p < N; p++) {
         for (int k = 0; k < 3; k++) {
             const float4 v = h[3 * p + k];
@@ -603,6 +1023,8 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
 struct rtx_batch {
     rtx_ctx* ctx = nullptr;
     rtx::Job* d_jobs = nullptr;
+    std::vector<rtx::Job> jobs;       // host copy of the job table
+    std::vector<uint64_t> handles;    // scene of each job
     rtx::Job first{};
     uint32_t njobs = 0, nwork = 0, tile_w = 8;
     bool use_graph = false;
@@ -628,6 +1050,8 @@ rtx_status rtx_batch_create(rtx_ctx* c, const rtx_lp_general* lps, uint32_t n, i
         built[i] = lookup(lp.is_handle);
         if (!built[i]) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_batch_create: unknown is_handle in frame " + std::to_string(i));
         if (built[i]->device != c->device) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_batch_create: handle on another device");
+        if (built[i]->two_level != built[0]->two_level)
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_create: flat and two-level scenes in one batch");
         if (lp.image_w == 0 || lp.image_h == 0 || lp.spp == 0 || lp.depth == 0 || !lp.rawRGB)
             return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_create: bad launch parameters in frame " + std::to_string(i));
         if ((uint64_t)lp.image_w * lp.image_h > 0xFFFFFFFFull / 2)
@@ -661,6 +1085,9 @@ rtx_status rtx_batch_create(rtx_ctx* c, const rtx_lp_general* lps, uint32_t n, i
     b->nwork = (uint32_t)work;
     b->tile_w = tw;
     b->first = jobs[0];
+    b->jobs = jobs;
+    b->handles.resize(n);
+    for (uint32_t k = 0; k < n; k++) b->handles[k] = lps[order[k]].is_handle;
     b->use_graph = use_graph != 0;
     hipError_t e = hipMalloc((void**)&b->d_jobs, sizeof(rtx::Job) * n);
     if (e == hipSuccess) e = hipMemcpy(b->d_jobs, jobs.data(), sizeof(rtx::Job) * n, hipMemcpyHostToDevice);
@@ -677,6 +1104,34 @@ rtx_status rtx_batch_launch(rtx_batch* b, void* stream) {
     if (!b) return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_launch: batch is NULL");
     HIP_OK(hipSetDevice(b->ctx->device));
     hipStream_t s = (hipStream_t)stream;
+    // a scene rebuilt by rtx_scene_update since the job table was written
+    // has new arrays: write the table again
+    bool stale = false;
+    uint32_t need = 1;
+    for (uint32_t k = 0; k < b->njobs; k++) {
+        Built* sc = lookup(b->handles[k]);
+        if (!sc) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_batch_launch: a frame's scene was destroyed");
+        if (sc->two_level != (b->first.sc.xinst != nullptr))
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_batch_launch: a frame's scene changed its build mode");
+        need = std::max(need, sc->stack_need);
+        if (std::memcmp(&b->jobs[k].sc, &sc->hdr, sizeof(rtx::DevScene)) != 0) {
+            b->jobs[k].sc = sc->hdr;
+            stale = true;
+        }
+    }
+    if (stale) {
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(rtx::Job) * b->njobs, hipMemcpyHostToDevice));
+        b->first = b->jobs[0];
+        if (rtx_status st = ensure_gstack(b->ctx->render_ls, need)) return st;
+        if (b->exec) {
+            // the captured kernel arguments hold the first job: capture again
+            (void)hipGraphExecDestroy(b->exec);
+            (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+        }
+    }
     if (!b->use_graph || !s) {
         hipError_t e = rtx::launch_render_batch(b->d_jobs, b->first, b->njobs, b->nwork, b->ctx->render_ls, b->tile_w, s);
         if (e != hipSuccess) return hip_fail(e, "rtx_batch_launch");

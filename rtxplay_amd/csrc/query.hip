@@ -12,7 +12,7 @@ namespace {
 // closest hit for a batch of rays (the optixTrace query on its own),
 // persistent grid with a grid-stride loop so that the stack overflow area is
 // sized by the grid, not by the batch
-template <int BLOCK>
+template <int BLOCK, bool TL>
 __global__ __launch_bounds__(BLOCK) void k_trace(const DevScene sc, const rtx_ray* __restrict__ rays, uint32_t n,
                                                  rtx_hit* __restrict__ hits, uint32_t* gstack, uint32_t gstride) {
     __shared__ uint32_t s_stack[kStackLDS * BLOCK];
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const DevScene sc, const rtx_ra
         TRay r;
         tray_init(r, f3(ray.ori.x, ray.ori.y, ray.ori.z), f3(ray.dir.x, ray.dir.y, ray.dir.z), ray.tmin);
         HitFull h;
-        traverse<BLOCK>(sc, r, ray.tmax, h, stk);
+        traverse<BLOCK, TL>(sc, r, ray.tmax, h, stk);
         rtx_hit o;
         if (h.tri == kNoHit) {
             o.t = __uint_as_float(0x7F800000u);
@@ -36,9 +36,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const DevScene sc, const rtx_ra
             o.t = h.t;
             o.u = h.u;
             o.v = h.v;
-            o.instance_id = __float_as_uint(tp[1].w);
+            o.instance_id = TL ? h.inst : __float_as_uint(tp[1].w);
             o.primitive_index = __float_as_uint(tp[2].w);
-            o.prim = h.prim;
+            // two-level: global id = the instance's first triangle + index in its mesh
+            o.prim = TL ? __float_as_uint(sc.xinst[(size_t)kInstF4 * h.inst + 6].y) + h.prim : h.prim;
         }
         o.pad[0] = o.pad[1] = 0u;
         hits[i] = o;
@@ -77,7 +78,7 @@ __global__ void k_pp(const float* __restrict__ src, uchar4* __restrict__ dst, ui
 
 uint32_t trace_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<kTraceBlock>, kTraceBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<kTraceBlock, false>, kTraceBlock, 0) != hipSuccess ||
         nb < 1)
         nb = 1;
     return (uint32_t)nb;
@@ -88,8 +89,12 @@ hipError_t launch_trace(const DevScene& sc, const rtx_ray* rays, uint32_t n, rtx
     if (n == 0) return hipSuccess;
     uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
     if (grid > ls.grid) grid = ls.grid;
-    hipLaunchKernelGGL(k_trace<kTraceBlock>, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, n, hits, ls.gstack,
-                       ls.gstride);
+    if (sc.xinst)
+        hipLaunchKernelGGL((k_trace<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, n, hits,
+                           ls.gstack, ls.gstride);
+    else
+        hipLaunchKernelGGL((k_trace<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, n, hits,
+                           ls.gstack, ls.gstride);
     return hipGetLastError();
 }
 

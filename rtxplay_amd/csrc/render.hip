@@ -85,7 +85,7 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
     return lo;
 }
 
-template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false>
+template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false>
 __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr) {
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
@@ -214,7 +214,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+        trav_run<BLOCK, COUNT, TL>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
                                wave_iters);
         if (COUNT) cyc_trav += clock64() - c0;
         if (!active || going) continue;
@@ -285,9 +285,17 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             // hit reconstruction, optx/optics_r.cu:45-90
             const float4* tp = sc.tris + 3 * (size_t)tv.h.tri;
             const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-            const uint32_t inst = __float_as_uint(t1.w);
+            const uint32_t inst = TL ? tv.h.inst : __float_as_uint(t1.w);
             const float4 m0 = sc.insts[2 * inst], m1 = sc.insts[2 * inst + 1];
-            const float3 A = xyz(t0), B = xyz(t1), C = xyz(t2);
+            float3 A = xyz(t0), B = xyz(t1), C = xyz(t2);
+            if (TL) {
+                // object-space vertices to world space (optx/optics_r.cu:59-75)
+                const float4* X = sc.xinst + (size_t)kInstF4 * inst;
+                const float4 w0 = X[3], w1 = X[4], w2 = X[5];
+                A = f3(xf_point(w0, A), xf_point(w1, A), xf_point(w2, A));
+                B = f3(xf_point(w0, B), xf_point(w1, B), xf_point(w2, B));
+                C = f3(xf_point(w0, C), xf_point(w1, C), xf_point(w2, C));
+            }
             const float w = 1.f - tv.h.u - tv.h.v;
             const float3 hp = (w * A + tv.h.u * B) + tv.h.v * C;
             float3 N = unitv(cross(B - A, C - A));
@@ -398,24 +406,24 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
 }
 
 // single frame: one block of 256 lanes per SIMD wave slot
-template <int BLOCK, bool COUNT, int SLDS>
+template <int BLOCK, bool COUNT, int SLDS, bool TL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    render_body<BLOCK, COUNT, false, SLDS>(a, s_stack);
+    render_body<BLOCK, COUNT, false, SLDS, false, TL>(a, s_stack);
 }
 
 // batch: per-lane job state needs more registers (4 waves per SIMD)
-template <int BLOCK, int SLDS>
+template <int BLOCK, int SLDS, bool TL>
 __global__ __launch_bounds__(BLOCK) void k_render_batch(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    render_body<BLOCK, false, true, SLDS>(a, s_stack);
+    render_body<BLOCK, false, true, SLDS, false, TL>(a, s_stack);
 }
 
 // adjoint: replays every sample path of a frame (same per-pixel Frand48
 // streams, same traversal) and back-propagates dL/d rawRGB along each
 // escaped path into per-instance d albedo and d fuzz; per-block LDS sums
 // (when the instances fit) are flushed with one global atomic per entry
-template <int BLOCK, int SLDS>
+template <int BLOCK, int SLDS, bool TL>
 __global__ __launch_bounds__(BLOCK) void k_adjoint(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
     extern __shared__ float s_grad[];
@@ -424,7 +432,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const RenderArgs a) {
         for (uint32_t i = threadIdx.x; i < ng; i += BLOCK) s_grad[i] = 0.f;
         __syncthreads();
     }
-    render_body<BLOCK, false, false, SLDS, true>(a, s_stack, s_grad);
+    render_body<BLOCK, false, false, SLDS, true, TL>(a, s_stack, s_grad);
     if (a.lds_grad) {
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < ng; i += BLOCK) {
@@ -459,6 +467,7 @@ __global__ void k_get_materials(const float4* __restrict__ insts, uint32_t n, fl
 
 // picker: one primary ray through (pick_x, pick_y), optx/camera_r.cu:26-32 and
 // optx/optics_r.cu:24-28 / optx/camera_r.cu:125-129
+template <bool TL>
 __global__ __launch_bounds__(64) void k_pick(const RenderArgs a) {
     __shared__ uint32_t s_stack[kStackLDS * 64];
     if (threadIdx.x != 0) return;
@@ -480,14 +489,29 @@ __global__ __launch_bounds__(64) void k_pick(const RenderArgs a) {
     TRay r;
     tray_init(r, o, d, 1e-3f);
     HitFull h;
-    traverse<64>(a.job.sc, r, 1e16f, h, stk);
-    *lp.pick_id = h.tri == kNoHit ? 0xFFFFFFFFu : __float_as_uint(a.job.sc.tris[3 * (size_t)h.tri + 1].w);
+    traverse<64, TL>(a.job.sc, r, 1e16f, h, stk);
+    *lp.pick_id = h.tri == kNoHit ? 0xFFFFFFFFu
+                                  : (TL ? h.inst : __float_as_uint(a.job.sc.tris[3 * (size_t)h.tri + 1].w));
 }
 
 uint32_t log2u(uint32_t v) {
     uint32_t l = 0;
     while ((1u << (l + 1)) <= v) l++;
     return l;
+}
+
+// the frame kernel for the scene's kind (flat / two-level) and the LDS stack size
+template <bool COUNT>
+hipError_t launch_frame(const RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
+    const dim3 g(ls.grid), b(kRenderBlock);
+    if (a.job.sc.xinst) {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, true>), g, b, 0, s, a);
+    } else {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, false>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, false>), g, b, 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 RenderArgs base_args(const LaunchScratch& ls) {
@@ -520,7 +544,7 @@ Job make_job(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, uint3
 
 uint32_t render_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false, kStackLDS>, kRenderBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false, kStackLDS, false>, kRenderBlock, 0) !=
             hipSuccess || nb < 1)
         nb = 1;
     return (uint32_t)nb;
@@ -533,11 +557,7 @@ hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 am
     a.njobs = 1;
     a.nwork = a.job.work_end;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-    if (ls.stack_lds == 16)
-        hipLaunchKernelGGL((k_render<kRenderBlock, false, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_render<kRenderBlock, false, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
-    return hipGetLastError();
+    return launch_frame<false>(a, ls, s);
 }
 
 hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, float3 ambient,
@@ -548,11 +568,7 @@ hipError_t launch_render_counted(const DevScene& sc, const rtx_lp_general& lp, f
     a.nwork = a.job.work_end;
     a.tc = tc;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-    if (ls.stack_lds == 16)
-        hipLaunchKernelGGL((k_render<kRenderBlock, true, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_render<kRenderBlock, true, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
-    return hipGetLastError();
+    return launch_frame<true>(a, ls, s);
 }
 
 hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njobs, uint32_t nwork,
@@ -564,10 +580,14 @@ hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njo
     a.njobs = njobs;
     a.nwork = nwork;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-    if (ls.stack_lds == 16)
-        hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32>), dim3(ls.grid), dim3(kRenderBlock), 0, s, a);
+    const dim3 g(ls.grid), b(kRenderBlock);
+    if (first.sc.xinst) {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32, true>), g, b, 0, s, a);
+    } else {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16, false>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32, false>), g, b, 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -587,10 +607,14 @@ hipError_t launch_adjoint(const DevScene& sc, const rtx_lp_general& lp, float3 a
     if (ng) hipLaunchKernelGGL(k_fill, dim3((ng + 255) / 256), dim3(256), 0, s, dparams, ng, 0.f);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     const size_t shm = a.lds_grad ? sizeof(float) * ng : 0;
-    if (ls.stack_lds == 16)
-        hipLaunchKernelGGL((k_adjoint<kRenderBlock, 16>), dim3(ls.grid), dim3(kRenderBlock), shm, s, a);
-    else
-        hipLaunchKernelGGL((k_adjoint<kRenderBlock, 32>), dim3(ls.grid), dim3(kRenderBlock), shm, s, a);
+    const dim3 g(ls.grid), b(kRenderBlock);
+    if (sc.xinst) {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_adjoint<kRenderBlock, 16, true>), g, b, shm, s, a);
+        else hipLaunchKernelGGL((k_adjoint<kRenderBlock, 32, true>), g, b, shm, s, a);
+    } else {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_adjoint<kRenderBlock, 16, false>), g, b, shm, s, a);
+        else hipLaunchKernelGGL((k_adjoint<kRenderBlock, 32, false>), g, b, shm, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -611,7 +635,8 @@ hipError_t launch_get_materials(const DevScene& sc, float* params, hipStream_t s
 hipError_t launch_pick(const DevScene& sc, const rtx_lp_general& lp, const LaunchScratch& ls, hipStream_t s) {
     RenderArgs a = base_args(ls);
     a.job = make_job(sc, lp, make_float3(0.f, 0.f, 0.f), 8u, 0);
-    hipLaunchKernelGGL(k_pick, dim3(1), dim3(64), 0, s, a);
+    if (sc.xinst) hipLaunchKernelGGL(k_pick<true>, dim3(1), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_pick<false>, dim3(1), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

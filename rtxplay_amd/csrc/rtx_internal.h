@@ -1,19 +1,28 @@
 // rtx_internal.h -- device data layouts and helpers shared by the kernels.
 //
 // Layout in HBM of a built scene (what a traversable handle points at):
-//   nodes : BVH2 traversal nodes, 64 B each = 4 x float4
-//           n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
-//           n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
-//           n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
-//           n3 = (ref c0, ref c1, -, -) as uint bits
-//           ref >= 0: inner node index; ref with bit 31 set: leaf,
-//           bits 27..30 = count-1, bits 0..26 = first leaf-ordered triangle.
+//   nodes : 4-wide BVH nodes, 128 B each = 8 x float4, structure of arrays
+//           over the 4 children: lo.x[4], hi.x[4], lo.y[4], hi.y[4],
+//           lo.z[4], hi.z[4], refs[4] (uint bits), unused
+//           ref without bit 31: inner node index; ref with bit 31 set: leaf,
+//           bits 27..30 = count-1, bits 0..26 = first leaf-ordered triangle;
+//           bits 27..31 all set (kInstLeaf): a top-level leaf holding one
+//           instance, whose id is in bits 0..26 (two-level scenes only).
+//           An empty child slot is a point box at 1e38 with a leaf ref to a
+//           degenerate triangle (or to the null instance of a top level).
 //   tris  : leaf-ordered triangles, 48 B each = 3 x float4
 //           (v0.xyz | global prim id), (v1.xyz | instance id),
 //           (v2.xyz | primitive index inside the instance's mesh)
+//           Two-level scenes hold each mesh's triangles once, in object
+//           space, and the ids are the mesh's own (instance 0, prim id =
+//           primitive index): the instance comes from the traversal.
 //   insts : 32 B per instance = 2 x float4
 //           (albedo.xyz | fuzz for metal, index for glass),
 //           (center.xyz = translation of the 3x4 transform | optics type)
+//   xinst : two-level scenes only, kInstF4 float4 per instance (+ the null
+//           instance of empty top-level slots): world-to-object 3x4 rows
+//           (3), object-to-world 3x4 rows (3), (bottom-level root, global id
+//           of the instance's first triangle, mesh, 0) as uint bits, unused
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,12 +44,16 @@ constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyRoot = 0xFFFFFFFFu;
 constexpr uint32_t kNoChild = 0x7FFFFFFFu;  // empty slot of a 4-wide node (no leaf flag, no valid node)
 constexpr int kWideNodeF4 = 8;             // float4 per 4-wide node (128 B)
+constexpr uint32_t kInstLeaf = 0xF8000000u; // top-level leaf of one instance: kInstLeaf | instance id
+constexpr uint32_t kInstMarker = 0x7FFFFFFEu;  // stack entry below an instance's bottom-level entries
+constexpr int kInstF4 = 8;                 // float4 per instance record of a two-level scene (128 B)
 
 // What a traversable handle points at (device memory); kernels get a copy by value.
 struct DevScene {
     const float4* nodes;
     const float4* tris;
     const float4* insts;
+    const float4* xinst;  // two-level scenes: instance transforms and roots; nullptr: flat scene
     uint32_t root;        // inner node 0, a leaf ref, or kEmptyRoot
     uint32_t num_prims;
     uint32_t num_nodes;

@@ -13,13 +13,20 @@
 // an edge compute exactly negated values) with an FP64 fallback when one is 0.
 // Closest hit: smallest t; an exact tie goes to the lower global primitive id,
 // so the result does not depend on traversal order.
+//
+// Two-level scenes (TL): the top level's leaves are instances.  Entering one
+// pushes the world ray (6 words) and a marker on the lane's stack,
+// transforms the ray into the instance's object space (the 3x4
+// world-to-object rows; the direction is not normalised, so t keeps its
+// meaning) and continues at the instance's bottom-level root; popping the
+// marker pops the world ray back.  A lane leaves an instance only with no
+// leaf parked, and parks no leaf at the top level, so a parked leaf is
+// always tested with the ray of its own instance.  Hits keep the instance
+// and the primitive index within its mesh; exact ties go to the lower
+// (instance, primitive), the order of the global primitive ids.
 #pragma once
 
 #include "rtx_internal.h"
-
-
-
-
 
 namespace rtx {
 
@@ -176,7 +183,8 @@ __device__ __forceinline__ void sort_ce(float& ka, uint32_t& ra, float& kb, uint
 struct HitFull {
     float t, u, v;
     uint32_t tri;    // leaf-ordered triangle, kNoHit on a miss
-    uint32_t prim;   // global primitive id
+    uint32_t prim;   // global primitive id (two-level scenes: primitive index within the instance's mesh)
+    uint32_t inst;   // instance of the hit (two-level scenes; flat scenes keep it in the triangle)
 };
 
 // Resumable traversal state of one ray: the ray, the next node to visit,
@@ -187,17 +195,70 @@ struct Trav {
     uint32_t node;    // next node to visit (kNoChild: none left)
     uint32_t leaf;    // a leaf found but not tested yet (kNoChild: none)
     int sp;
+    uint32_t inst;    // two-level scenes: the instance the lane is in (kNoHit: top level)
 };
 
-__device__ __forceinline__ void trav_begin(Trav& s, const DevScene& sc, float3 o, float3 d, float tmin, float tmax) {
-    tray_init(s.r, o, d, tmin);
+__device__ __forceinline__ void trav_reset(Trav& s, const DevScene& sc, float tmax) {
     s.h.t = tmax;
     s.h.u = s.h.v = 0.f;
     s.h.tri = kNoHit;
     s.h.prim = kNoHit;
+    s.h.inst = kNoHit;
     s.node = sc.root;
     s.leaf = kNoChild;
     s.sp = 0;
+    s.inst = kNoHit;
+}
+
+__device__ __forceinline__ void trav_begin(Trav& s, const DevScene& sc, float3 o, float3 d, float tmin, float tmax) {
+    tray_init(s.r, o, d, tmin);
+    trav_reset(s, sc, tmax);
+}
+
+// row . (p, 1) and row . (v, 0) of a 3x4 transform, one rounding per operation
+// (the order of the flattening build, so that a two-level scene's world
+// vertices are bit for bit those of the flat scene)
+__device__ __forceinline__ float xf_point(const float4& row, float3 p) {
+    return ((p.x * row.x + p.y * row.y) + p.z * row.z) + row.w;
+}
+__device__ __forceinline__ float xf_vector(const float4& row, float3 v) {
+    return (v.x * row.x + v.y * row.y) + v.z * row.z;
+}
+
+// two-level scenes: enters the instance of top-level leaf `ref` (the lane
+// is at the top level, no leaf parked)
+template <class Stack>
+__device__ __forceinline__ void inst_enter(const DevScene& sc, Trav& s, uint32_t ref, Stack& st) {
+    const uint32_t id = ref & kLeafFirstMask;
+    const float4* X = sc.xinst + (size_t)kInstF4 * id;
+    const float4 r0 = X[0], r1 = X[1], r2 = X[2], m = X[6];
+    const float3 wo = s.r.o, wd = s.r.d;
+    st.push(s.sp, __float_as_uint(wo.x));
+    st.push(s.sp, __float_as_uint(wo.y));
+    st.push(s.sp, __float_as_uint(wo.z));
+    st.push(s.sp, __float_as_uint(wd.x));
+    st.push(s.sp, __float_as_uint(wd.y));
+    st.push(s.sp, __float_as_uint(wd.z));
+    st.push(s.sp, kInstMarker);
+    const float3 o = f3(xf_point(r0, wo), xf_point(r1, wo), xf_point(r2, wo));
+    const float3 d = f3(xf_vector(r0, wd), xf_vector(r1, wd), xf_vector(r2, wd));
+    tray_init(s.r, o, d, s.r.tmin);
+    s.inst = id;
+    s.node = __float_as_uint(m.x);
+}
+
+// two-level scenes: leaves the instance whose marker the lane just popped
+template <class Stack>
+__device__ __forceinline__ void inst_leave(Trav& s, Stack& st) {
+    float3 wo, wd;
+    wd.z = __uint_as_float(st.pop(s.sp));
+    wd.y = __uint_as_float(st.pop(s.sp));
+    wd.x = __uint_as_float(st.pop(s.sp));
+    wo.z = __uint_as_float(st.pop(s.sp));
+    wo.y = __uint_as_float(st.pop(s.sp));
+    wo.x = __uint_as_float(st.pop(s.sp));
+    tray_init(s.r, wo, wd, s.r.tmin);
+    s.inst = kNoHit;
 }
 
 // first active lane of the wave (counts wave-level loop iterations in COUNT mode)
@@ -221,7 +282,7 @@ __device__ __forceinline__ bool first_active_lane() {
 // least `batch` of the wave's `busy` lanes have finished, so that the caller
 // can shade them while the others keep their state and resume later.
 // COUNT adds node-visit and triangle-test counts (statistics launches only).
-template <int BLOCK, bool COUNT = false, class Stack>
+template <int BLOCK, bool COUNT = false, bool TL = false, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
                                          int inner_frac, Stack& st, uint32_t* nn = nullptr,
                                          uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
@@ -229,6 +290,15 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
     for (;;) {
         // node phase
         for (;;) {
+            if (TL && going) {
+                // leave a finished instance (not while its leaf is parked),
+                // then enter the instance the lane stands on, if any
+                if (s.node == kInstMarker && s.leaf == kNoChild) {
+                    inst_leave(s, st);
+                    s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
+                }
+                if ((s.node & kInstLeaf) == kInstLeaf) inst_enter(sc, s, s.node, st);
+            }
             // a lane that reached a leaf parks it (one slot) and goes on with
             // the next node from its stack, so it keeps stepping while the
             // others do; the parked leaves are tested together below
@@ -236,7 +306,7 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 s.leaf = s.node;
                 s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
             }
-            const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild;
+            const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild && (!TL || s.node != kInstMarker);
             const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
             if (n_inner == 0) break;
             if (n_inner * 64u < (uint32_t)inner_frac * (uint32_t)__popcll(__ballot(going))) break;
@@ -310,12 +380,15 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 float tt, uu, vv;
                 if (watertight(s.r, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv)) {
                     const uint32_t prim = __float_as_uint(t0.w);
-                    if (tt < s.h.t || (tt == s.h.t && prim < s.h.prim)) {
+                    const bool lower = TL ? (s.inst < s.h.inst || (s.inst == s.h.inst && prim < s.h.prim))
+                                          : prim < s.h.prim;
+                    if (tt < s.h.t || (tt == s.h.t && lower)) {
                         s.h.t = tt;
                         s.h.u = uu;
                         s.h.v = vv;
                         s.h.tri = k;
                         s.h.prim = prim;
+                        if (TL) s.h.inst = s.inst;
                     }
                 }
             }
@@ -328,20 +401,13 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
 }
 
 // closest hit, run to completion (query kernels and the picker)
-template <int BLOCK, bool COUNT = false, class Stack>
-__device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, float tmax, HitFull& h,
-                                         Stack& st, uint32_t* nn = nullptr, uint32_t* nt = nullptr) {
+template <int BLOCK, bool TL = false, class Stack>
+__device__ __forceinline__ void traverse(const DevScene& sc, const TRay& r, float tmax, HitFull& h, Stack& st) {
     Trav s;
     s.r = r;
-    s.h.t = tmax;
-    s.h.u = s.h.v = 0.f;
-    s.h.tri = kNoHit;
-    s.h.prim = kNoHit;
-    s.node = sc.root;
-    s.leaf = kNoChild;
-    s.sp = 0;
+    trav_reset(s, sc, tmax);
     bool going = true;
-    trav_run<BLOCK, COUNT>(sc, s, going, true, 64, 32, st, nn, nt);
+    trav_run<BLOCK, false, TL>(sc, s, going, true, 64, 32, st);
     h = s.h;
 }
 

@@ -89,10 +89,13 @@ class RTOWScene(Scene):
 
     level: an int (every sphere at that subdivision level) or "mixed" (the
     reference's per-sphere levels); spec: a sphere_things-style list to load
-    instead of the seeded one."""
+    instead of the seeded one; two_level: build the reference's layout (one
+    structure per mesh under an instance structure) instead of the flat one."""
 
-    def __init__(self, seed=0, half=11, level=4, device=0, spec=None):
+    def __init__(self, seed=0, half=11, level=4, device=0, spec=None, two_level=False):
         super().__init__(device)
+        if two_level:
+            self.build_mode(Scene.TWO_LEVEL)
         self.seed, self.half, self.level = seed, half, level
         self.things_spec = []
         self._spec = spec
@@ -156,12 +159,12 @@ class BatchFrames:
     context and acceleration structure, per frame), one Launcher per frame
     for its output buffers, and the LpGeneral records."""
 
-    def __init__(self, frames, device=0, spp_scale=1.0):
+    def __init__(self, frames, device=0, spp_scale=1.0, two_level=False):
         from .api import Camera, Launcher, LpGeneral
         self.frames = frames
         self.scenes, self.launchers, self.lps = [], [], []
         for f in frames:
-            sc = RTOWScene(seed=f["seed"], half=f["half"], level=f["level"], device=device)
+            sc = RTOWScene(seed=f["seed"], half=f["half"], level=f["level"], device=device, two_level=two_level)
             sc.load()
             handle = sc.build()
             lp = LpGeneral()
