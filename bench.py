@@ -266,6 +266,57 @@ def interactive_leg(frames=120):
     return rec
 
 
+def two_level_leg(spp=64, steps=3):
+    """Flat vs two-level layout (the reference's IAS over one GAS per mesh,
+    optx/scene.cxx:38-294) on the level-6 scene and on the reference's
+    mixed-level scene (per-sphere levels 9/8/6/3, optx/rtwo.cxx:137-244),
+    1920x1080, depth 16, at `spp` per frame: build time, device bytes,
+    rays/s, and for the two-level scene the refit (Scene::update) after
+    every small sphere moved."""
+    import rtxplay_amd as rtx
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene, transform_for
+
+    w, h = 1920, 1080
+    rec = {"workload": "rtow-final-scene, 1920x1080, %d spp, depth 16, seed 0" % spp}
+    for level in (6, "mixed"):
+        for tl in (False, True):
+            sc = RTOWScene(seed=0, half=11, level=level, two_level=tl)
+            sc.load()
+            lp = rtx.LpGeneral()
+            lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, 16, sc.build()
+            st = sc.stats()
+            lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0).c
+            la = rtx.Launcher(sc, lp)
+            stream = Stream()
+            la.ignite(stream=stream)
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            for _ in range(steps):
+                la.ignite(stream=stream, sync=False)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / steps
+            rays = int(la.rays_per_pixel().sum(dtype=np.uint64))
+            r = {"instanced_triangles": st["num_prims"], "stored_triangles": st["unique_prims"],
+                 "build_ms": round(st["build_ms"], 1), "device_bytes": st["device_bytes"],
+                 "ms_per_frame": round(ms, 2), "mrays": round(rays / (ms * 1e-3) / 1e6, 1)}
+            if tl:
+                for k, t in enumerate(sc.things_spec):
+                    if t["kind"] == 1:
+                        m = dict(t)
+                        m["center"] = (m["center"][0], np.float32(m["center"][1] + .05), m["center"][2])
+                        sc.set(k, transform_for(m))
+                sc.update(sc.handle)
+                r["refit_ms"] = round(sc.stats()["build_ms"], 3)
+            key = "level%s_%s" % (level, "two_level" if tl else "flat")
+            log("%s: %s" % (key, r))
+            rec[key] = r
+            la.free()
+            sc.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,9 +327,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
-    ap.add_argument("--legs", default="frame,batch,diff,interactive",
+    ap.add_argument("--legs", default="frame,batch,diff,interactive,two_level",
                     help="comma list of: frame (the headline render), batch (64 frames), diff (adjoint), "
-                         "interactive (repeated 1-spp launches)")
+                         "interactive (repeated 1-spp launches), two_level (flat vs two-level layouts)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -318,6 +369,7 @@ def main():
         log("batched speedup %.2fx" % batched["speedup"])
     diff = diff_leg(max(3, args.batch_reps)) if "diff" in legs else None
     inter = interactive_leg() if "interactive" in legs else None
+    two = two_level_leg() if "two_level" in legs else None
 
     cpu = None
     if not args.no_cpu and fr is not None:
@@ -340,7 +392,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
         "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
-        "interactive": inter,
+        "interactive": inter, "two_level": two,
     }
     print(json.dumps(out), flush=True)
 

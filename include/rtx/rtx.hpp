@@ -164,6 +164,14 @@ class Scene {
     void update(TraversableHandle is_handle, void* stream = nullptr) {
         check(rtx_scene_update(ctx_, is_handle, stream), "rtx_scene_update");
     }
+    // layout of the next build: RTX_BUILD_FLAT (default) or RTX_BUILD_TWO_LEVEL
+    // (the reference's IAS over one GAS per mesh, optx/scene.cxx:38-273)
+    void build_mode(int mode) { check(rtx_scene_set_build_mode(ctx_, mode), "rtx_scene_set_build_mode"); }
+    int build_mode() {
+        int m = 0;
+        check(rtx_scene_get_build_mode(ctx_, &m), "rtx_scene_get_build_mode");
+        return m;
+    }
     rtx_ctx* context() { return ctx_; }
 
     // launch tuning (no reference counterpart); fields left 0 keep their value

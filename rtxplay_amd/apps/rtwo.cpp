@@ -2,7 +2,10 @@
 // batch path of optx/rtwo.cxx:78-620 with the OptiX pipeline replaced by the
 // C-ABI (through include/rtx/rtx.hpp).
 //
-//   rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-S] [-q] [-A RPP] > out.ppm
+//   rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-I] [-S] [-q] [-A RPP] > out.ppm
+//
+// -I builds the reference's two-level layout (an instance structure over
+// one structure per mesh) instead of the flat one.
 //
 // Output: PPM P3 on stdout, top row first (optx/rtwo.cxx:622-637); with -A RPP
 // also a PGM P2 of rays per pixel (optx/rtwo.cxx:656-666); with -S the line
@@ -109,7 +112,7 @@ class RTWO : public rtx::Scene {
 };
 
 void usage() {
-    std::cerr << "usage: rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-S] [-q] [-A RPP]\n";
+    std::cerr << "usage: rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-I] [-S] [-q] [-A RPP]\n";
 }
 
 }  // namespace
@@ -118,7 +121,7 @@ int main(int argc, char* argv[]) {
     unsigned w = 1280, h = 720, spp = 50, depth = 16;
     int level = -1;
     uint32_t seed = 0;
-    bool stats = false, quiet = false, aov_rpp = false;
+    bool stats = false, quiet = false, aov_rpp = false, two_level = false;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto val = [&]() -> std::string {
@@ -133,6 +136,7 @@ int main(int argc, char* argv[]) {
         else if (a == "-l") { const std::string l = val(); level = l == "mixed" ? -1 : std::atoi(l.c_str()); }
         else if (a == "-r") seed = (uint32_t)std::strtoul(val().c_str(), nullptr, 10);
         else if (a == "-S") stats = true;
+        else if (a == "-I") two_level = true;
         else if (a == "-q") quiet = true;
         else if (a == "-A") aov_rpp = val() == "RPP";
         else if (a == "-h") { usage(); return 0; }
@@ -149,6 +153,7 @@ int main(int argc, char* argv[]) {
         lp.camera = camera.pod();
 
         RTWO rtwo(seed, level);
+        if (two_level) rtwo.build_mode(RTX_BUILD_TWO_LEVEL);
         rtwo.load();
         rtwo.build(&lp.is_handle);
 

@@ -321,17 +321,19 @@ def test_postprocess_matches_oracle(rtx, orc):
 
 
 # ---- the C++ batch driver over the same library ------------------------------------
-def test_cpp_rtwo_driver_matches_python_path(rtx):
+@pytest.mark.parametrize("two_level", [False, True])
+def test_cpp_rtwo_driver_matches_python_path(rtx, two_level):
     """rtxplay_amd/rtwo (include/rtx/rtx.hpp over the C-ABI) renders the same
-    bytes as the Python mirror for the same seeded scene."""
+    bytes as the Python mirror for the same seeded scene, flat and (-I)
+    two-level."""
     import subprocess
     from conftest import ROOT
     from rtxplay_amd.scene import RTOWScene
     exe = os.path.join(ROOT, "rtxplay_amd", "rtwo")
     w, h, spp, depth = 48, 27, 3, 16
-    say("rtwo driver", w, h, spp)
-    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", str(spp), "-d", str(depth), "-l", "2", "-A", "RPP", "-S"],
-                       capture_output=True, text=True, timeout=120)
+    say("rtwo driver", w, h, spp, "two-level", two_level)
+    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", str(spp), "-d", str(depth), "-l", "2", "-A", "RPP", "-S"] +
+                       (["-I"] if two_level else []), capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
     tok = p.stdout.split()
     assert tok[0] == "P3" and int(tok[1]) == w and int(tok[2]) == h
@@ -340,7 +342,7 @@ def test_cpp_rtwo_driver_matches_python_path(rtx):
     assert tok[k] == "P2"
     pgm = np.array(tok[k + 4:k + 4 + w * h], np.int64).reshape(h, w)[::-1]
     assert "(pixel, rays, milliseconds)" in p.stderr
-    s = RTOWScene(seed=0, half=11, level=2)
+    s = RTOWScene(seed=0, half=11, level=2, two_level=two_level)
     s.load()
     s.build()
     L, lp = _launch(rtx, s, w, h, spp, depth)
