@@ -412,11 +412,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDE
     render_body<BLOCK, COUNT, false, SLDS, false, TL>(a, s_stack);
 }
 
-// batch: per-lane job state needs more registers (4 waves per SIMD)
+// batch: the same loop with a job per lane; flat scenes fit 4 waves per
+// SIMD without spills, two-level ones take the registers of 3
 template <int BLOCK, int SLDS, bool TL>
-__global__ __launch_bounds__(BLOCK) void k_render_batch(const RenderArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render_batch(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    render_body<BLOCK, false, true, SLDS, false, TL>(a, s_stack);
+    render_body<BLOCK, false, true, SLDS, false, false>(a, s_stack);
+}
+template <int BLOCK, int SLDS>
+__global__ __launch_bounds__(BLOCK) void k_render_batch_tl(const RenderArgs a) {
+    __shared__ uint32_t s_stack[SLDS * BLOCK];
+    render_body<BLOCK, false, true, SLDS, false, true>(a, s_stack);
 }
 
 // adjoint: replays every sample path of a frame (same per-pixel Frand48
@@ -582,8 +588,8 @@ hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njo
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     const dim3 g(ls.grid), b(kRenderBlock);
     if (first.sc.xinst) {
-        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32, true>), g, b, 0, s, a);
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render_batch_tl<kRenderBlock, 16>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render_batch_tl<kRenderBlock, 32>), g, b, 0, s, a);
     } else {
         if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render_batch<kRenderBlock, 16, false>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_render_batch<kRenderBlock, 32, false>), g, b, 0, s, a);
