@@ -68,6 +68,54 @@ def cpu_baseline(level, w, h, spp, depth, seconds, threads):
                       % (rows, w, h, spp, depth, level, dt, rays)}
 
 
+def cpu_reference(seconds):
+    """The reference's own CPU tracer, rtow.cxx as built from its sources
+    (oracle/_ref/rtow: FP64, analytic spheres, 1280x720, 10 spp, depth 50,
+    one thread, as written), for `seconds`: it reports each row it starts
+    on stderr, so the rows finished and the time they took are exact; the
+    rays of those rows come from the oracle's restatement, which
+    reproduces the tracer's image byte for byte.  None when the binary is
+    not built."""
+    import select
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "rtow")
+    if not os.path.exists(exe):
+        return None
+    import oracle
+    w, h = 1280, 720
+    p = subprocess.Popen([exe], stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    t0 = time.perf_counter()
+    buf, starts = b"", []
+    try:
+        while time.perf_counter() - t0 < seconds:
+            r, _, _ = select.select([p.stderr], [], [], 0.5)
+            if not r:
+                if p.poll() is not None:
+                    break
+                continue
+            chunk = os.read(p.stderr.fileno(), 4096)
+            if not chunk:
+                break
+            now = time.perf_counter() - t0
+            buf += chunk
+            parts = buf.split(b"\r")
+            buf = parts[-1]
+            for tok in parts[:-1]:
+                if tok.strip():
+                    starts.append(now)
+    finally:
+        p.kill()
+        p.wait()
+    done = max(len(starts) - 1, 0)          # rows finished: a later row has started
+    if done == 0:
+        return None
+    dt = starts[done]
+    rays = oracle.rtow_rays(done * w, w, h, 10, 50)
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "reference",
+            "sample": "rtow.cxx built from the reference sources (g++ -O2): its first %d rows of 1280x720 at "
+                      "10 spp, depth 50, FP64 analytic spheres (%.1f s, %d rays)" % (done, dt, rays)}
+
+
 TUNED_JSON = os.path.join(ROOT, "rtxplay_amd", "tuned.json")
 ARCH = "gfx950"
 
@@ -371,11 +419,14 @@ def main():
     inter = interactive_leg() if "interactive" in legs else None
     two = two_level_leg() if "two_level" in legs else None
 
-    cpu = None
+    cpu = cpu_ref = None
     if not args.no_cpu and fr is not None:
         threads = max(1, min(16, os.cpu_count() or 1))
         cpu = cpu_baseline(level, w, h, spp, depth, args.cpu_seconds, threads)
         log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
+        cpu_ref = cpu_reference(args.cpu_seconds)
+        if cpu_ref:
+            log("reference cpu tracer: %.2f Mrays/s on 1 thread" % cpu_ref["value"])
 
     fr = fr or {}
     out = {
@@ -389,7 +440,7 @@ def main():
         "samples_per_s": fr.get("samples_per_s"), "Msamples_per_s": fr.get("Msamples_per_s"),
         "rays_per_frame": fr.get("rays"), "rays_per_sample": fr.get("rays_per_sample"),
         "bvh": fr.get("bvh"), "roofline": fr.get("roofline"),
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "cpu_reference": cpu_ref,
         "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
         "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
         "interactive": inter, "two_level": two,

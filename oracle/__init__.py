@@ -71,6 +71,7 @@ def lib():
             "orc_render_frozen": (None, [POINTER(SceneC), POINTER(CameraC), c_uint32, c_uint32, c_uint32, c_uint32,
                                          c_uint32, c_uint32, c_int, P, P]),
             "orc_rtow_render": (c_uint64, [c_int, c_int, c_int, c_int, c_int, P]),
+            "orc_rtow_rays": (c_uint64, [c_int, c_int, c_int, c_int, c_int, c_uint64]),
             "orc_rtow_scene": (c_uint32, [c_int, P, c_uint32]),
             "orc_pp_srgb": (None, [P, P, c_int, c_int]),
             "orc_pp_srgb_f64": (None, [P, P, c_int, c_int]),
@@ -319,6 +320,11 @@ def rtow_render(w=1280, h=720, spp=10, depth=50, argorder=1):
     out = np.empty((h, w, 3), np.uint8)
     rays = lib().orc_rtow_render(w, h, spp, depth, argorder, _p(out))
     return out, rays
+
+
+def rtow_rays(npix, w=1280, h=720, spp=10, depth=50, argorder=1):
+    """Rays the reference tracer casts for its first npix output pixels."""
+    return int(lib().orc_rtow_rays(w, h, spp, depth, argorder, int(npix)))
 
 
 def pp_srgb(raw):

@@ -1242,7 +1242,9 @@ static d3 dtrace(const dthings* t, d3 ro, d3 rd, int depth) {
 
 static inline double dclamp(double x, double lo, double hi) { return lo > x ? lo : x > hi ? hi : x; }
 
-uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t* rgb) {
+/* the first npix pixels in the reference's output order (rows from the top);
+   rgb may be NULL (ray count only) */
+static uint64_t rtow_render_n(int w, int h, int spp, int depth, int argorder, uint8_t* rgb, uint64_t npix) {
     g_r2l = argorder;
     srand(1);
     dthings things = rtow_scene();
@@ -1261,8 +1263,9 @@ uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t
     d3 dvec = dscl(fostance, cw);
     g_rays = 0;
     size_t o = 0;
-    for (int y = h - 1; y >= 0; --y) {                 /* rtow.cxx:105-121 */
-        for (int x = 0; x < w; ++x) {
+    uint64_t done = 0;
+    for (int y = h - 1; y >= 0 && done < npix; --y) {  /* rtow.cxx:105-121 */
+        for (int x = 0; x < w && done < npix; ++x, ++done) {
             d3 color = dk(0, 0, 0);
             for (int k = 0; k < spp; ++k) {
                 double s = 2. * (x + rrnd()) / (w - 1) - 1.;
@@ -1273,6 +1276,7 @@ uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t
                 d3 rd = dsub(dsub(dadd(dscl(s, wvec), dscl(t, hvec)), dvec), off);
                 color = dadd(color, dtrace(&things, ro, rd, depth));
             }
+            if (!rgb) continue;
             d3 c = ddiv(color, spp);                                  /* rtow.cxx:119 */
             double r = sqrt(c.x), g = sqrt(c.y), b = sqrt(c.z);       /* rtow.cxx:13 */
             rgb[o++] = (uint8_t)(int)(256 * dclamp(r, 0, .999));
@@ -1282,6 +1286,14 @@ uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t
     }
     free(things.s);
     return g_rays;
+}
+
+uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t* rgb) {
+    return rtow_render_n(w, h, spp, depth, argorder, rgb, (uint64_t)w * (uint64_t)h);
+}
+
+uint64_t orc_rtow_rays(int w, int h, int spp, int depth, int argorder, uint64_t npix) {
+    return rtow_render_n(w, h, spp, depth, argorder, NULL, npix);
 }
 
 /* ========================================================================= */

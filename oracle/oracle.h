@@ -154,6 +154,8 @@ void orc_render_frozen(const orc_scene* s, const orc_camera* cam, uint32_t w, ui
  * (the reference leaves the order to the compiler).  Writes w*h*3 bytes in
  * PPM order (top row first).  Returns rays traced.  Calls srand(1) first. */
 uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t* rgb);
+/* rays traced for the first npix pixels of that image (its output order) */
+uint64_t orc_rtow_rays(int w, int h, int spp, int depth, int argorder, uint64_t npix);
 /* the rtow scene (rtow.cxx:51-80) as things, same argorder convention */
 uint32_t orc_rtow_scene(int argorder, orc_thing* out, uint32_t cap);
 
