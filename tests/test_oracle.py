@@ -23,6 +23,21 @@ def test_rtow_restatement_bit_exact(orc):
     assert rays > img.shape[0] * img.shape[1] * int(g["spp"])
 
 
+def test_rtow_ray_count_of_a_prefix(orc):
+    """orc_rtow_rays (the benchmark's ray count for the reference tracer's
+    finished rows) counts the rays of the first pixels of the same render:
+    the whole image gives the render's count, a row more adds at least one
+    ray per sample of that row, and the count does not depend on pixels
+    after the prefix."""
+    w, h, spp, depth = 48, 27, 2, 50
+    _, rays = orc.rtow_render(w, h, spp, depth, argorder=1)
+    assert orc.rtow_rays(w * h, w, h, spp, depth) == rays
+    counts = [orc.rtow_rays(r * w, w, h, spp, depth) for r in range(h + 1)]
+    assert counts[0] == 0 and counts[-1] == rays
+    assert all(b - a >= w * spp for a, b in zip(counts, counts[1:]))
+    assert orc.rtow_rays(5 * w, w, h, spp, depth) == orc.rtow_rays(5 * w, w, h, spp, depth)
+
+
 def test_rtow_argorder_matters(orc):
     """The other argument-evaluation order gives another scene (so the exact
     match above really pins the order)."""
