@@ -25,7 +25,10 @@ namespace rtx {
 
 namespace {
 
-constexpr int kBins = 32;
+#ifndef RTX_SAH_BINS
+#define RTX_SAH_BINS 64
+#endif
+constexpr int kBins = RTX_SAH_BINS;
 constexpr int kBuildBlock = 256;
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)

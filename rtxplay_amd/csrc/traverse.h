@@ -208,6 +208,7 @@ struct Trav {
     HitFull h;
     uint32_t node;    // next node to visit (kNoChild: none left)
     uint32_t leaf;    // a leaf found but not tested yet (kNoChild: none)
+    uint32_t leaf2;   // a second one (set only while leaf is)
     int sp;
     uint32_t inst;    // two-level scenes: the instance the lane is in (kNoHit: top level)
 };
@@ -220,6 +221,7 @@ __device__ __forceinline__ void trav_reset(Trav& s, const DevScene& sc, float tm
     s.h.inst = kNoHit;
     s.node = sc.root;
     s.leaf = kNoChild;
+    s.leaf2 = kNoChild;
     s.sp = 0;
     s.inst = kNoHit;
 }
@@ -285,10 +287,10 @@ __device__ __forceinline__ bool first_active_lane() {
 // `going` when its ray is done (closest hit final in s.h).
 //
 // Each round has a node phase and a leaf phase.  The node phase steps the
-// lanes that sit on an inner node; a lane that reaches a leaf parks it in a
-// one-leaf slot and goes on with the next node from its stack (Aila and
-// Laine 2009, "speculative traversal"), and waits only when it reaches a
-// second leaf or runs out of nodes.  The phase ends when fewer than
+// lanes that sit on an inner node; a lane that reaches a leaf parks it in
+// one of two leaf slots and goes on with the next node from its stack (Aila
+// and Laine 2009, "speculative traversal"), and waits only when it reaches a
+// third leaf or runs out of nodes.  The phase ends when fewer than
 // inner_frac/64 of the going lanes can still step; the leaf phase then tests
 // the parked leaves of all lanes together.  inner_frac = 1 runs the node
 // phase until no lane can step.  The function
@@ -313,11 +315,12 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 }
                 if ((s.node & kInstLeaf) == kInstLeaf) inst_enter(sc, s, s.node, st);
             }
-            // a lane that reached a leaf parks it (one slot) and goes on with
-            // the next node from its stack, so it keeps stepping while the
-            // others do; the parked leaves are tested together below
-            if (going && s.leaf == kNoChild && (s.node & kLeafFlag)) {
-                s.leaf = s.node;
+            // a lane that reached a leaf parks it (two slots) and goes on
+            // with the next node from its stack, so it keeps stepping while
+            // the others do; the parked leaves are tested together below
+            if (going && (s.node & kLeafFlag) && (s.leaf == kNoChild || s.leaf2 == kNoChild)) {
+                if (s.leaf == kNoChild) s.leaf = s.node;
+                else s.leaf2 = s.node;
                 s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
             }
             const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild && (!TL || s.node != kInstMarker);
@@ -392,8 +395,14 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             s.leaf = kNoChild;
             const uint32_t first = lf & kLeafFirstMask;
             const uint32_t cnt = ((lf >> kLeafCountShift) & 15u) + 1u;
-            if (COUNT) *nt += cnt;
-            for (uint32_t k = first; k < first + cnt; k++) {
+            const uint32_t lf2 = s.leaf2;
+            s.leaf2 = kNoChild;
+            const uint32_t first2 = lf2 & kLeafFirstMask;
+            const uint32_t cnt2 = lf2 == kNoChild ? 0u : ((lf2 >> kLeafCountShift) & 15u) + 1u;
+            if (COUNT) *nt += cnt + cnt2;
+            // the triangles of both parked leaves in one loop
+            for (uint32_t q = 0; q < cnt + cnt2; q++) {
+                const uint32_t k = q < cnt ? first + q : first2 + (q - cnt);
                 if (COUNT && first_active_lane()) wi[1]++;
                 // (32-bit byte offsets: the builder keeps the triangle array below 4 GiB)
                 const char* tb = (const char*)sc.tris;
