@@ -2,20 +2,37 @@
 // batch path of optx/rtwo.cxx:78-620 with the OptiX pipeline replaced by the
 // C-ABI (through include/rtx/rtx.hpp).
 //
-//   rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-I] [-S] [-q] [-A RPP] > out.ppm
+//   rtwo [-g WxH|NAME] [-a W:H] [-s spp] [-d depth] [-A RPP] [-G] [-D TYPE] [-S] [-q] [-v] [-t] [-h]
+//        [-l level|mixed] [-r seed] [-I] > out.ppm
 //
-// -I builds the reference's two-level layout (an instance structure over
-// one structure per mesh) instead of the flat one.
+// The reference's batch options (optx/args.cxx:23-121) with their long
+// forms: -g/--geometry takes WxH or a named resolution (optx/args.h:20-42),
+// -a/--aspect-ratio W:H sets the height from the width, -s/--samples-per-pixel,
+// -d/--trace-depth, -A/--print-aov RPP, -G/--print-guides, -S/--print-statistics,
+// -q/--quiet/--silent, -h/--help/--usage, -v/--verbose and -t/--trace-sm
+// (accepted; the latter only matters to the interactive viewer).
+// -D/--apply-denoiser TYPE is accepted, but the OptiX denoiser is not part
+// of this build: the image stays as rendered, and -G prints the guides
+// (normals, albedos) only with -D, as the reference does.
+// Options of this driver only: -l the subdivision level of every sphere
+// (default: the reference's mixed levels), -r the scene seed, -I the
+// reference's two-level layout (an instance structure over one structure
+// per mesh) instead of the flat one.
 //
-// Output: PPM P3 on stdout, top row first (optx/rtwo.cxx:622-637); with -A RPP
-// also a PGM P2 of rays per pixel (optx/rtwo.cxx:656-666); with -S the line
-// "pixels rays milliseconds fps" on stderr (optx/rtwo.cxx:578-591).
+// Output on stdout, top row first: the guides as PPM P3 (with -D -G,
+// optx/rtwo.cxx:554-558, 639-654), the image as PPM P3
+// (optx/rtwo.cxx:622-637), rays per pixel as PGM P2 with -A RPP
+// (optx/rtwo.cxx:656-666); with -S the line "pixels rays milliseconds fps"
+// on stderr (optx/rtwo.cxx:578-591).
+#include <getopt.h>
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -112,36 +129,117 @@ class RTWO : public rtx::Scene {
 };
 
 void usage() {
-    std::cerr << "usage: rtwo [-g WxH] [-s spp] [-d depth] [-l level|mixed] [-r seed] [-I] [-S] [-q] [-A RPP]\n";
+    std::cerr << "usage: rtwo [-g WxH|NAME] [-a W:H] [-s spp] [-d depth] [-A RPP] [-G] [-D TYPE] [-S] [-q] [-v] [-t] [-h]\n"
+                 "            [-l level|mixed] [-r seed] [-I]\n";
+}
+
+// named resolutions of the -g option (optx/args.h:20-42)
+const std::map<std::string, std::pair<int, int>> kResolutions = {
+    {"CGA", {320, 200}},     {"HVGA", {480, 320}},    {"VGA", {640, 480}},     {"WVGA", {800, 480}},
+    {"SVGA", {800, 600}},    {"XGA", {1024, 768}},    {"HD", {1280, 720}},     {"SXGA", {1280, 1024}},
+    {"UXGA", {1600, 1200}},  {"FULLHD", {1920, 1080}}, {"2K", {2048, 1080}},   {"QXGA", {2048, 1536}},
+    {"UWHD", {2560, 1080}},  {"WQHD", {2560, 1440}},  {"WQXGA", {2560, 1600}}, {"UWQHD", {3440, 1440}},
+    {"UHD-1", {3840, 2160}}, {"4K", {4096, 2160}},    {"5K-UW", {5120, 2160}}, {"5K", {5120, 2880}},
+    {"UHD-2", {7680, 4320}}};
+
+// float3 image as PPM P3, top row first, components clamped to [0, 1] and
+// scaled by 255 with truncation (optx/rtwo.cxx:639-654)
+void print_float3(const std::vector<rtx::float3>& img, unsigned w, unsigned h) {
+    std::cout << "P3\n" << w << ' ' << h << '\n' << 255 << '\n';
+    auto c = [](float v) { return (int)((v < 0.f ? 0.f : v > 1.f ? 1.f : v) * 255); };
+    for (int y = (int)h - 1; y >= 0; --y)
+        for (unsigned x = 0; x < w; ++x) {
+            const rtx::float3 p = img[(size_t)w * y + x];
+            std::cout << c(p.x) << ' ' << c(p.y) << ' ' << c(p.z) << '\n';
+        }
+    std::cout << std::endl;
 }
 
 }  // namespace
 
 int main(int argc, char* argv[]) {
-    unsigned w = 1280, h = 720, spp = 50, depth = 16;
+    // optx/rtwo.cxx:89-98: 1280x720, 50 spp, depth 16 unless given
+    int gw = -1, gh = -1;
+    unsigned spp = 50, depth = 16;
     int level = -1;
     uint32_t seed = 0;
-    bool stats = false, quiet = false, aov_rpp = false, two_level = false;
-    for (int i = 1; i < argc; i++) {
-        const std::string a = argv[i];
-        auto val = [&]() -> std::string {
-            if (i + 1 >= argc) { usage(); std::exit(1); }
-            return argv[++i];
-        };
-        if (a == "-g") {
-            const std::string g = val();
-            if (std::sscanf(g.c_str(), "%ux%u", &w, &h) != 2 || !w || !h) { usage(); return 1; }
-        } else if (a == "-s") spp = (unsigned)std::atoi(val().c_str());
-        else if (a == "-d") depth = (unsigned)std::atoi(val().c_str());
-        else if (a == "-l") { const std::string l = val(); level = l == "mixed" ? -1 : std::atoi(l.c_str()); }
-        else if (a == "-r") seed = (uint32_t)std::strtoul(val().c_str(), nullptr, 10);
-        else if (a == "-S") stats = true;
-        else if (a == "-I") two_level = true;
-        else if (a == "-q") quiet = true;
-        else if (a == "-A") aov_rpp = val() == "RPP";
-        else if (a == "-h") { usage(); return 0; }
-        else { usage(); return 1; }
+    bool stats = false, quiet = false, aov_rpp = false, two_level = false, guides = false, denoise = false;
+    int flag_v = 0, flag_t = 0, flag_q = 0, flag_h = 0, flag_G = 0, flag_S = 0;
+    const struct option lopts[] = {
+        {"geometry", required_argument, nullptr, 'g'},     {"aspect-ratio", required_argument, nullptr, 'a'},
+        {"samples-per-pixel", required_argument, nullptr, 's'}, {"trace-depth", required_argument, nullptr, 'd'},
+        {"verbose", no_argument, &flag_v, 1},              {"trace-sm", no_argument, &flag_t, 1},
+        {"quiet", no_argument, &flag_q, 1},                {"silent", no_argument, &flag_q, 1},
+        {"help", no_argument, &flag_h, 1},                 {"usage", no_argument, &flag_h, 1},
+        {"print-aov", required_argument, nullptr, 'A'},    {"print-guides", no_argument, &flag_G, 1},
+        {"print-statistics", no_argument, &flag_S, 1},     {"apply-denoiser", required_argument, nullptr, 'D'},
+        {nullptr, 0, nullptr, 0}};
+    int c;
+    while ((c = getopt_long(argc, argv, "g:a:s:d:vtqhA:GSD:l:r:I", lopts, nullptr)) != -1) {
+        switch (c) {
+            case 0: break;   // a flag set through lopts
+            case 'g': {
+                const auto it = kResolutions.find(optarg);
+                if (it != kResolutions.end()) {
+                    gw = it->second.first;
+                    gh = it->second.second;
+                } else if (std::sscanf(optarg, "%dx%d", &gw, &gh) != 2 || gw < 1 || gh < 1) {
+                    gw = gh = -1;   // optx/args.cxx:30-34: a bad geometry keeps the default
+                }
+                break;
+            }
+            case 'a': {
+                float aw = 0.f, ah = 0.f;   // optx/args.cxx:40-45
+                if (std::sscanf(optarg, "%f:%f", &aw, &ah) == 2 && gw > 0 && aw > 0.f && ah > 0.f)
+                    gh = (int)((float)gw * ah / aw + .5f);
+                break;
+            }
+            case 's': spp = (unsigned)std::abs(std::atoi(optarg)); break;
+            case 'd': depth = (unsigned)std::abs(std::atoi(optarg)); break;
+            case 'v': flag_v = 1; break;
+            case 't': flag_t = 1; break;
+            case 'q': flag_q = 1; break;
+            case 'h': flag_h = 1; break;
+            case 'G': flag_G = 1; break;
+            case 'S': flag_S = 1; break;
+            case 'A': {
+                // comma-separated AOV names (optx/args.cxx:64-80)
+                std::string list = optarg;
+                size_t pos = 0;
+                while (pos <= list.size()) {
+                    const size_t e = list.find(',', pos);
+                    const std::string aov = list.substr(pos, e == std::string::npos ? std::string::npos : e - pos);
+                    if (aov == "RPP") aov_rpp = true;
+                    else std::cerr << "rtwo: unknown argument for option A ignored -- " << aov << std::endl;
+                    if (e == std::string::npos) break;
+                    pos = e + 1;
+                }
+                break;
+            }
+            case 'D': {
+                const std::string t = optarg;
+                if (t == "SMP" || t == "NRM" || t == "ALB" || t == "NAA" || t == "AOV") {
+                    denoise = true;
+                    std::cerr << "rtwo: the OptiX denoiser is not part of this build; the image is not denoised"
+                              << std::endl;
+                } else {
+                    std::cerr << "rtwo: unknown argument for option D ignored -- " << t << std::endl;
+                }
+                break;
+            }
+            case 'l': level = std::string(optarg) == "mixed" ? -1 : std::atoi(optarg); break;
+            case 'r': seed = (uint32_t)std::strtoul(optarg, nullptr, 10); break;
+            case 'I': two_level = true; break;
+            default: usage(); return 1;   // optx/args.cxx:96-98
+        }
     }
+    if (flag_h) { usage(); return 0; }
+    (void)flag_v;
+    (void)flag_t;
+    quiet = flag_q != 0;
+    stats = flag_S != 0;
+    guides = flag_G != 0 && denoise;
+    const unsigned w = gw > 0 ? (unsigned)gw : 1280u, h = gh > 0 ? (unsigned)gh : 720u;
     try {
         rtx::LpGeneral lp{};
         lp.image_w = w;
@@ -172,6 +270,13 @@ int main(int argc, char* argv[]) {
         std::vector<uint32_t> rpp(n);
         rtx::check(rtx_memcpy_dtoh(rpp.data(), lp.rpp, sizeof(uint32_t) * n), "rtx_memcpy_dtoh");
 
+        if (!quiet && guides) {
+            std::vector<rtx::float3> g(n);
+            rtx::check(rtx_memcpy_dtoh(g.data(), lp.normals, sizeof(rtx::float3) * n), "rtx_memcpy_dtoh");
+            print_float3(g, w, h);
+            rtx::check(rtx_memcpy_dtoh(g.data(), lp.albedos, sizeof(rtx::float3) * n), "rtx_memcpy_dtoh");
+            print_float3(g, w, h);
+        }
         if (!quiet) {
             std::cout << "P3\n" << w << ' ' << h << '\n' << 255 << '\n';
             for (int y = (int)h - 1; y >= 0; --y)

@@ -353,3 +353,47 @@ def test_cpp_rtwo_driver_matches_python_path(rtx, two_level):
     assert np.array_equal(ppm, img[..., :3].astype(np.int64))
     assert np.array_equal(pgm, L.rays_per_pixel().astype(np.int64))
     out.free(); L.free(); s.close()
+
+
+def test_cpp_rtwo_reference_options(rtx):
+    """The reference's batch options (optx/args.cxx:23-121): a named
+    resolution with -a, long options, -A with a list, and the guides that
+    -G prints only together with -D (optx/rtwo.cxx:549-558)."""
+    import subprocess
+    from conftest import ROOT
+    from rtxplay_amd.scene import RTOWScene
+    exe = os.path.join(ROOT, "rtxplay_amd", "rtwo")
+    say("rtwo -g CGA -a 16:9")
+    p = subprocess.run([exe, "-g", "CGA", "-a", "16:9", "-s", "1", "-d", "4", "-l", "2", "-q", "-S"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == ""
+    assert p.stderr.split()[0] == str(320 * 180)              # 320 x int(320 * 9 / 16 + .5)
+    w, h, spp = 40, 24, 2
+    say("rtwo long options, -D NRM -G -A RPP,XYZ")
+    p = subprocess.run([exe, "--geometry", "%dx%d" % (w, h), "--samples-per-pixel", str(spp), "--trace-depth", "16",
+                        "-l", "2", "--apply-denoiser", "NRM", "--print-guides", "--print-aov", "RPP,XYZ"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "unknown argument for option A ignored -- XYZ" in p.stderr
+    tok = p.stdout.split()
+    imgs, k = [], 0
+    while k < len(tok):
+        magic, iw, ih, mx = tok[k], int(tok[k + 1]), int(tok[k + 2]), int(tok[k + 3])
+        n = iw * ih * (3 if magic == "P3" else 1)
+        imgs.append((magic, mx, np.array(tok[k + 4:k + 4 + n], np.int64)))
+        k += 4 + n
+    assert [m for m, _, _ in imgs] == ["P3", "P3", "P3", "P2"]
+    s = RTOWScene(seed=0, half=11, level=2)
+    s.load()
+    s.build()
+    L, lp = _launch(rtx, s, w, h, spp, 16)
+    want_n = (np.clip(L.normals(), 0, 1) * 255).astype(np.int64)[::-1].reshape(-1)
+    want_a = (np.clip(L.albedos(), 0, 1) * 255).astype(np.int64)[::-1].reshape(-1)
+    assert np.array_equal(imgs[0][2], want_n) and np.array_equal(imgs[1][2], want_a)
+    assert np.array_equal(imgs[3][2], L.rays_per_pixel()[::-1].reshape(-1).astype(np.int64))
+    # without -D the guides are not printed
+    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", "1", "-l", "2", "-G"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0 and p.stdout.split().count("P3") == 1
+    L.free(); s.close()
