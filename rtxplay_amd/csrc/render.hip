@@ -32,6 +32,9 @@
 #ifndef RTX_RENDER_WAVES
 #define RTX_RENDER_WAVES 4
 #endif
+#ifndef RTX_BATCH_WAVES
+#define RTX_BATCH_WAVES RTX_RENDER_WAVES
+#endif
 
 namespace rtx {
 
@@ -85,7 +88,7 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
     return lo;
 }
 
-template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false>
+template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false, bool FMA = true>
 __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr) {
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
@@ -214,7 +217,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT, TL>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+        trav_run<BLOCK, COUNT, TL, FMA>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
                                wave_iters);
         if (COUNT) cyc_trav += clock64() - c0;
         if (!active || going) continue;
@@ -412,17 +415,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDE
     render_body<BLOCK, COUNT, false, SLDS, false, TL>(a, s_stack);
 }
 
-// batch: the same loop with a job per lane; flat scenes fit 4 waves per
-// SIMD without spills, two-level ones take the registers of 3
+// batch: the same loop with a job per lane, whose job state takes
+// registers: the slab test in its (P - o) * idir form keeps flat scenes at
+// 4 waves per SIMD (7 % faster than the FMA form under that register
+// limit, and 20 % faster than 3 waves); two-level ones take the registers
+// of 3 waves
 template <int BLOCK, int SLDS, bool TL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render_batch(const RenderArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_BATCH_WAVES))) void k_render_batch(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    render_body<BLOCK, false, true, SLDS, false, false>(a, s_stack);
+    render_body<BLOCK, false, true, SLDS, false, false, false>(a, s_stack);
 }
 template <int BLOCK, int SLDS>
 __global__ __launch_bounds__(BLOCK) void k_render_batch_tl(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    render_body<BLOCK, false, true, SLDS, false, true>(a, s_stack);
+    render_body<BLOCK, false, true, SLDS, false, true, false>(a, s_stack);
 }
 
 // adjoint: replays every sample path of a frame (same per-pixel Frand48

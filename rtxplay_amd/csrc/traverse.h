@@ -47,14 +47,12 @@ struct TRay {
     // through on each axis (lo when the direction component is >= 0, else
     // hi); the exit planes are at offset ^ 16
     uint32_t nx, ny, nz;
-#ifndef RTX_SLAB_SUBMUL
-    // slab distances as one FMA per plane, P * idir - c: c = o * idir
+    // for slab distances as one FMA per plane, P * idir - c: c = o * idir
     // moved by s = 2^-22 |o * idir| (down for entry planes, up for exit
     // planes), which covers the roundings of o * idir and of c +- s, so
     // an entry distance never comes out above (P - o) * idir rounded
     // once, nor an exit distance below it
     float3 ce, cx;
-#endif
 };
 
 
@@ -84,12 +82,10 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     r.nx = r.idir.x >= 0.f ? 0u : 16u;
     r.ny = r.idir.y >= 0.f ? 32u : 48u;
     r.nz = r.idir.z >= 0.f ? 64u : 80u;
-#ifndef RTX_SLAB_SUBMUL
     const float3 c = f3(o.x * r.idir.x, o.y * r.idir.y, o.z * r.idir.z);
     const float3 sl = f3(fabsf(c.x) * 0x1p-22f, fabsf(c.y) * 0x1p-22f, fabsf(c.z) * 0x1p-22f);
     r.ce = f3(c.x + sl.x, c.y + sl.y, c.z + sl.z);
     r.cx = f3(c.x - sl.x, c.y - sl.y, c.z - sl.z);
-#endif
 }
 
 // true with t > tmin and barycentrics (u,v) of v1 and v2 on a hit
@@ -298,7 +294,10 @@ __device__ __forceinline__ bool first_active_lane() {
 // least `batch` of the wave's `busy` lanes have finished, so that the caller
 // can shade them while the others keep their state and resume later.
 // COUNT adds node-visit and triangle-test counts (statistics launches only).
-template <int BLOCK, bool COUNT = false, bool TL = false, class Stack>
+// FMA selects the slab form: one FMA per plane with the per-ray offsets
+// (fewer instructions), or (P - o) * idir (six fewer registers, for kernels
+// that are short of them).
+template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
                                          int inner_frac, Stack& st, uint32_t* nn = nullptr,
                                          uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
@@ -349,18 +348,22 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             int nhit = 0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-#ifdef RTX_SLAB_SUBMUL
-                const float x0 = (comp4(ex, j) - r.o.x) * r.idir.x, x1 = (comp4(fx, j) - r.o.x) * r.idir.x;
-                const float y0 = (comp4(ey, j) - r.o.y) * r.idir.y, y1 = (comp4(fy, j) - r.o.y) * r.idir.y;
-                const float z0 = (comp4(ez, j) - r.o.z) * r.idir.z, z1 = (comp4(fz, j) - r.o.z) * r.idir.z;
-#else
-                const float x0 = __builtin_fmaf(comp4(ex, j), r.idir.x, -r.ce.x);
-                const float x1 = __builtin_fmaf(comp4(fx, j), r.idir.x, -r.cx.x);
-                const float y0 = __builtin_fmaf(comp4(ey, j), r.idir.y, -r.ce.y);
-                const float y1 = __builtin_fmaf(comp4(fy, j), r.idir.y, -r.cx.y);
-                const float z0 = __builtin_fmaf(comp4(ez, j), r.idir.z, -r.ce.z);
-                const float z1 = __builtin_fmaf(comp4(fz, j), r.idir.z, -r.cx.z);
-#endif
+                float x0, x1, y0, y1, z0, z1;
+                if (FMA) {
+                    x0 = __builtin_fmaf(comp4(ex, j), r.idir.x, -r.ce.x);
+                    x1 = __builtin_fmaf(comp4(fx, j), r.idir.x, -r.cx.x);
+                    y0 = __builtin_fmaf(comp4(ey, j), r.idir.y, -r.ce.y);
+                    y1 = __builtin_fmaf(comp4(fy, j), r.idir.y, -r.cx.y);
+                    z0 = __builtin_fmaf(comp4(ez, j), r.idir.z, -r.ce.z);
+                    z1 = __builtin_fmaf(comp4(fz, j), r.idir.z, -r.cx.z);
+                } else {
+                    x0 = (comp4(ex, j) - r.o.x) * r.idir.x;
+                    x1 = (comp4(fx, j) - r.o.x) * r.idir.x;
+                    y0 = (comp4(ey, j) - r.o.y) * r.idir.y;
+                    y1 = (comp4(fy, j) - r.o.y) * r.idir.y;
+                    z0 = (comp4(ez, j) - r.o.z) * r.idir.z;
+                    z1 = (comp4(fz, j) - r.o.z) * r.idir.z;
+                }
                 const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
                 const float tf = fminf(fminf(x1, y1), fminf(z1, s.h.t));
                 // empty slots hold a point box at 1e38 that no slab test accepts
