@@ -466,3 +466,36 @@ def test_two_level_adjoint_matches_finite_differences(rtx):
     s.set_materials(m0)
     L.free()
     s.close()
+
+
+_DET_SCRIPT = r'''
+import hashlib, sys
+sys.path.insert(0, sys.argv[1])
+import rtxplay_amd as rtx
+from rtxplay_amd.scene import RTOWScene
+s = RTOWScene(seed=0, half=11, level=4, two_level=True)
+s.load()
+s.build()
+lp = rtx.LpGeneral()
+lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = 96, 54, 16, 16, s.handle
+lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, 96 / 54, 0.1, 10.0).c
+L = rtx.Launcher(s, lp)
+L.ignite()
+print(hashlib.sha1(L.raw_rgb().tobytes() + L.rays_per_pixel().tobytes()).hexdigest())
+'''
+
+
+def test_two_level_frame_is_the_same_in_every_process(rtx):
+    """The frame does not depend on which lanes take which pixels (the work
+    counter hands them out in a different order every launch): two fresh
+    processes render the same bytes."""
+    import subprocess
+    import sys
+    from conftest import ROOT
+    outs = []
+    for k in range(2):
+        say("fresh process", k)
+        p = subprocess.run([sys.executable, "-c", _DET_SCRIPT, ROOT], capture_output=True, text=True, timeout=180)
+        assert p.returncode == 0, p.stderr
+        outs.append(p.stdout.strip().split()[-1])
+    assert outs[0] == outs[1]
