@@ -92,9 +92,14 @@ struct Frand48 {
     }
     __device__ __forceinline__ float next() {
         state = (0x5DEECE66Dull * state + 0xBull) & 0xFFFFFFFFFFFFull;
-        const float hi = (float)(uint32_t)(state >> 24) * 0x1p-24f;     // exact
-        const float lo = (float)(uint32_t)(state & 0xFFFFFFu) * 0x1p-48f; // exact
-        return __fadd_rn(hi, lo);
+        // bits 24..47 by a 32-bit funnel shift of the two state words: as a
+        // 64-bit shift the compiler converts the (provably 32-bit) value
+        // with the multi-instruction 64-bit integer-to-float sequence
+        const uint32_t w0 = (uint32_t)state, w1 = (uint32_t)(state >> 32);
+        const float hi = (float)__builtin_amdgcn_alignbit(w1, w0, 24);  // exact
+        const float lo = (float)(w0 & 0xFFFFFFu);                        // exact
+        // RN(hi + lo * 2^-24) in one rounding, then an exact scale by 2^-24
+        return __builtin_fmaf(lo, 0x1p-24f, hi) * 0x1p-24f;
     }
     // util::rnd(min,max): optx/util_gpu.h:28
     __device__ __forceinline__ float range(float lo, float hi) { return lo + next() * (hi - lo); }
