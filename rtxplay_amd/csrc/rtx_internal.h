@@ -47,6 +47,12 @@ constexpr int kWideNodeF4 = 8;             // float4 per 4-wide node (128 B)
 constexpr uint32_t kInstLeaf = 0xF8000000u; // top-level leaf of one instance: kInstLeaf | instance id
 constexpr uint32_t kInstMarker = 0x7FFFFFFEu;  // stack entry below an instance's bottom-level entries
 constexpr int kInstF4 = 8;                 // float4 per instance record of a two-level scene (128 B)
+#ifndef RTX_TL_FRAC
+#define RTX_TL_FRAC 16
+#endif
+// two-level traversal: instance transitions run once this many 64ths of a
+// wave's traversing lanes wait for one (or when the node phase ends)
+constexpr int kTlFrac = RTX_TL_FRAC;
 
 // What a traversable handle points at (device memory); kernels get a copy by value.
 struct DevScene {

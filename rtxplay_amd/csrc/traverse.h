@@ -305,19 +305,36 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
     for (;;) {
         // node phase
         for (;;) {
-            if (TL && going) {
-                // leave a finished instance (not while its leaf is parked),
-                // then enter the instance the lane stands on, if any
-                if (s.node == kInstMarker && s.leaf == kNoChild) {
-                    inst_leave(s, st);
-                    s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
+            if (TL) {
+                // Instance transitions: leave a finished instance (not while
+                // one of its leaves is parked), then enter the instance the
+                // lane stands on, if any.  A transition costs several node
+                // steps of instructions, so a lane that needs one waits until
+                // kTlFrac/64 of the going lanes do, or until the node phase
+                // would end without them, and they transition together.
+                const bool leave = going && s.node == kInstMarker && s.leaf == kNoChild;
+                const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
+                const uint64_t need = __ballot(leave || enter);
+                if (need) {
+                    const uint32_t n_go = (uint32_t)__popcll(__ballot(going));
+                    const bool in0 = going && !(s.node & kLeafFlag) && s.node != kNoChild && s.node != kInstMarker;
+                    const uint32_t n_in = (uint32_t)__popcll(__ballot(in0));
+                    if ((uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * n_go ||
+                        n_in * 64u < (uint32_t)inner_frac * n_go || n_in == 0) {
+                        if (leave) {
+                            inst_leave(s, st);
+                            s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
+                        }
+                        if (going && (s.node & kInstLeaf) == kInstLeaf) inst_enter(sc, s, s.node, st);
+                    }
                 }
-                if ((s.node & kInstLeaf) == kInstLeaf) inst_enter(sc, s, s.node, st);
             }
             // a lane that reached a leaf parks it (two slots) and goes on
             // with the next node from its stack, so it keeps stepping while
             // the others do; the parked leaves are tested together below
-            if (going && (s.node & kLeafFlag) && (s.leaf == kNoChild || s.leaf2 == kNoChild)) {
+            // (an instance leaf waits for its transition instead)
+            if (going && (s.node & kLeafFlag) && (!TL || (s.node & kInstLeaf) != kInstLeaf) &&
+                (s.leaf == kNoChild || s.leaf2 == kNoChild)) {
                 if (s.leaf == kNoChild) s.leaf = s.node;
                 else s.leaf2 = s.node;
                 s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
