@@ -143,6 +143,33 @@ struct LaneStack {
         else v = glob[(size_t)(sp - SLDS) * gstride];
         return v;
     }
+    // pushes v[0..N-1] (v[N-1] ends on top) with one range check when all N
+    // slots are in LDS
+    template <int N>
+    __device__ __forceinline__ void push_n(int& sp, const uint32_t (&v)[N]) {
+        if (sp + N <= SLDS) {
+            lds_u32* p = lds + sp * BLOCK;
+#pragma unroll
+            for (int i = 0; i < N; i++) p[i * BLOCK] = v[i];
+            sp += N;
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; i++) push(sp, v[i]);
+        }
+    }
+    // pops N entries into v[0..N-1] (v[N-1] is the old top), the inverse of push_n
+    template <int N>
+    __device__ __forceinline__ void pop_n(int& sp, uint32_t (&v)[N]) {
+        if (sp <= SLDS) {
+            sp -= N;
+            const lds_u32* p = lds + sp * BLOCK;
+#pragma unroll
+            for (int i = 0; i < N; i++) v[i] = p[i * BLOCK];
+        } else {
+#pragma unroll
+            for (int i = N - 1; i >= 0; i--) v[i] = pop(sp);
+        }
+    }
     // writes slot sp (the free slot above the top) without pushing
     __device__ __forceinline__ void put(int sp, uint32_t v) {
         if (sp < SLDS) lds[sp * BLOCK] = v;
@@ -245,13 +272,9 @@ __device__ __forceinline__ void inst_enter(const DevScene& sc, Trav& s, uint32_t
     const float4* X = sc.xinst + (size_t)kInstF4 * id;
     const float4 r0 = X[0], r1 = X[1], r2 = X[2], m = X[6];
     const float3 wo = s.r.o, wd = s.r.d;
-    st.push(s.sp, __float_as_uint(wo.x));
-    st.push(s.sp, __float_as_uint(wo.y));
-    st.push(s.sp, __float_as_uint(wo.z));
-    st.push(s.sp, __float_as_uint(wd.x));
-    st.push(s.sp, __float_as_uint(wd.y));
-    st.push(s.sp, __float_as_uint(wd.z));
-    st.push(s.sp, kInstMarker);
+    const uint32_t w[7] = {__float_as_uint(wo.x), __float_as_uint(wo.y), __float_as_uint(wo.z), __float_as_uint(wd.x),
+                           __float_as_uint(wd.y), __float_as_uint(wd.z), kInstMarker};
+    st.push_n(s.sp, w);
     const float3 o = f3(xf_point(r0, wo), xf_point(r1, wo), xf_point(r2, wo));
     const float3 d = f3(xf_vector(r0, wd), xf_vector(r1, wd), xf_vector(r2, wd));
     tray_init(s.r, o, d, s.r.tmin);
@@ -262,13 +285,10 @@ __device__ __forceinline__ void inst_enter(const DevScene& sc, Trav& s, uint32_t
 // two-level scenes: leaves the instance whose marker the lane just popped
 template <class Stack>
 __device__ __forceinline__ void inst_leave(Trav& s, Stack& st) {
-    float3 wo, wd;
-    wd.z = __uint_as_float(st.pop(s.sp));
-    wd.y = __uint_as_float(st.pop(s.sp));
-    wd.x = __uint_as_float(st.pop(s.sp));
-    wo.z = __uint_as_float(st.pop(s.sp));
-    wo.y = __uint_as_float(st.pop(s.sp));
-    wo.x = __uint_as_float(st.pop(s.sp));
+    uint32_t w[6];
+    st.pop_n(s.sp, w);
+    const float3 wo = f3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
+    const float3 wd = f3(__uint_as_float(w[3]), __uint_as_float(w[4]), __uint_as_float(w[5]));
     tray_init(s.r, wo, wd, s.r.tmin);
     s.inst = kNoHit;
 }
