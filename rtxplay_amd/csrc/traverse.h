@@ -74,11 +74,13 @@ __device__ __forceinline__ void tray_init(TRay& r, float3 o, float3 d, float tmi
     r.rz0 = kz == 0;
     r.rz1 = kz == 1;
     const float3 dp = permute(r, d);       // (d[kx], d[ky], d[kz])
-    r.Sz = 1.f / dz;
-    r.Sx = dp.x * r.Sz;                     // shear by the reciprocal: one division
-    r.Sy = dp.y * r.Sz;
     r.idir = f3(1.f / (d.x != 0.f ? d.x : 1e-30f), 1.f / (d.y != 0.f ? d.y : 1e-30f),
                 1.f / (d.z != 0.f ? d.z : 1e-30f));
+    // Sz = 1 / d[kz]: that is idir[kz] unless d is the zero vector
+    const float iz = r.rz0 ? r.idir.x : (r.rz1 ? r.idir.y : r.idir.z);
+    r.Sz = dz != 0.f ? iz : __builtin_copysignf(__builtin_inff(), dz);
+    r.Sx = dp.x * r.Sz;                     // shear by the reciprocal: no division of its own
+    r.Sy = dp.y * r.Sz;
     r.nx = r.idir.x >= 0.f ? 0u : 16u;
     r.ny = r.idir.y >= 0.f ? 32u : 48u;
     r.nz = r.idir.z >= 0.f ? 64u : 80u;
@@ -264,33 +266,39 @@ __device__ __forceinline__ float xf_vector(const float4& row, float3 v) {
     return (v.x * row.x + v.y * row.y) + v.z * row.z;
 }
 
-// two-level scenes: enters the instance of top-level leaf `ref` (the lane
-// is at the top level, no leaf parked)
+// two-level scenes: one instance transition of the lanes that call it.  A
+// lane with `leave` set has just popped its instance's marker (and has no
+// leaf parked): it pops the world ray and the next entry.  A going lane that
+// then stands on an instance leaf enters that instance: it pushes the world
+// ray and a marker and continues at the instance's bottom-level root with
+// the ray in object space.  The lanes that left, entered or both set their
+// new ray up together, in one call.
 template <class Stack>
-__device__ __forceinline__ void inst_enter(const DevScene& sc, Trav& s, uint32_t ref, Stack& st) {
-    const uint32_t id = ref & kLeafFirstMask;
-    const float4* X = sc.xinst + (size_t)kInstF4 * id;
-    const float4 r0 = X[0], r1 = X[1], r2 = X[2], m = X[6];
-    const float3 wo = s.r.o, wd = s.r.d;
-    const uint32_t w[7] = {__float_as_uint(wo.x), __float_as_uint(wo.y), __float_as_uint(wo.z), __float_as_uint(wd.x),
-                           __float_as_uint(wd.y), __float_as_uint(wd.z), kInstMarker};
-    st.push_n(s.sp, w);
-    const float3 o = f3(xf_point(r0, wo), xf_point(r1, wo), xf_point(r2, wo));
-    const float3 d = f3(xf_vector(r0, wd), xf_vector(r1, wd), xf_vector(r2, wd));
-    tray_init(s.r, o, d, s.r.tmin);
-    s.inst = id;
-    s.node = __float_as_uint(m.x);
-}
-
-// two-level scenes: leaves the instance whose marker the lane just popped
-template <class Stack>
-__device__ __forceinline__ void inst_leave(Trav& s, Stack& st) {
-    uint32_t w[6];
-    st.pop_n(s.sp, w);
-    const float3 wo = f3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
-    const float3 wd = f3(__uint_as_float(w[3]), __uint_as_float(w[4]), __uint_as_float(w[5]));
-    tray_init(s.r, wo, wd, s.r.tmin);
-    s.inst = kNoHit;
+__device__ __forceinline__ void inst_step(const DevScene& sc, Trav& s, bool going, bool leave, Stack& st) {
+    float3 o = s.r.o, d = s.r.d;
+    if (leave) {
+        uint32_t w[6];
+        st.pop_n(s.sp, w);
+        o = f3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
+        d = f3(__uint_as_float(w[3]), __uint_as_float(w[4]), __uint_as_float(w[5]));
+        s.inst = kNoHit;
+        s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
+    }
+    const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
+    if (enter) {
+        const uint32_t id = s.node & kLeafFirstMask;
+        const float4* X = sc.xinst + (size_t)kInstF4 * id;
+        const float4 r0 = X[0], r1 = X[1], r2 = X[2], m = X[6];
+        const uint32_t w[7] = {__float_as_uint(o.x), __float_as_uint(o.y), __float_as_uint(o.z), __float_as_uint(d.x),
+                               __float_as_uint(d.y), __float_as_uint(d.z), kInstMarker};
+        st.push_n(s.sp, w);
+        const float3 wo = o, wd = d;
+        o = f3(xf_point(r0, wo), xf_point(r1, wo), xf_point(r2, wo));
+        d = f3(xf_vector(r0, wd), xf_vector(r1, wd), xf_vector(r2, wd));
+        s.inst = id;
+        s.node = __float_as_uint(m.x);
+    }
+    if (leave || enter) tray_init(s.r, o, d, s.r.tmin);
 }
 
 // first active lane of the wave (counts wave-level loop iterations in COUNT mode)
@@ -329,25 +337,14 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 // Instance transitions: leave a finished instance (not while
                 // one of its leaves is parked), then enter the instance the
                 // lane stands on, if any.  A transition costs several node
-                // steps of instructions, so a lane that needs one waits until
-                // kTlFrac/64 of the going lanes do, or until the node phase
-                // would end without them, and they transition together.
+                // steps of instructions, so a lane that needs one waits
+                // (without stepping) until kTlFrac/64 of the going lanes do,
+                // or until the round's transition step after the leaf phase.
                 const bool leave = going && s.node == kInstMarker && s.leaf == kNoChild;
                 const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
                 const uint64_t need = __ballot(leave || enter);
-                if (need) {
-                    const uint32_t n_go = (uint32_t)__popcll(__ballot(going));
-                    const bool in0 = going && !(s.node & kLeafFlag) && s.node != kNoChild && s.node != kInstMarker;
-                    const uint32_t n_in = (uint32_t)__popcll(__ballot(in0));
-                    if ((uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * n_go ||
-                        n_in * 64u < (uint32_t)inner_frac * n_go || n_in == 0) {
-                        if (leave) {
-                            inst_leave(s, st);
-                            s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
-                        }
-                        if (going && (s.node & kInstLeaf) == kInstLeaf) inst_enter(sc, s, s.node, st);
-                    }
-                }
+                if (need && (uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * (uint32_t)__popcll(__ballot(going)))
+                    inst_step(sc, s, going, leave, st);
             }
             // a lane that reached a leaf parks it (two slots) and goes on
             // with the next node from its stack, so it keeps stepping while
@@ -464,6 +461,13 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                     }
                 }
             }
+        }
+        if (TL) {
+            // the round's transition step: every lane waiting for one (no
+            // leaf is parked after the leaf phase)
+            const bool leave = going && s.node == kInstMarker;
+            const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
+            if (__ballot(leave || enter)) inst_step(sc, s, going, leave, st);
         }
         if (going && s.node == kNoChild && s.leaf == kNoChild) going = false;
         const uint64_t g = __ballot(going);
