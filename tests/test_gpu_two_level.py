@@ -314,6 +314,29 @@ def test_two_level_render_matches_flat(rtx, level4_pair):
     assert raw.min() >= 0 and raw.max() <= 1
 
 
+def test_two_level_tuning_does_not_change_pixels(rtx, level4_pair):
+    """Instance transitions run when enough of a wave waits for one, and the
+    world ray and marker go to the LDS part of the stack or its global
+    overflow area, so the launch tuning changes when and where transitions
+    happen.  The closest hit does not depend on it: every tuning gives the
+    same frame bit for bit (a 16-entry LDS stack sends the transitions of
+    the deeper instances through the overflow area)."""
+    _, t, _, _ = level4_pair
+    old = t.tuning()
+    ref = _render(rtx, t, 80, 45, 8, 16)
+    try:
+        for tun in (dict(stack_lds=16), dict(stack_lds=16, shade_batch=8, inner_frac=48),
+                    dict(shade_batch=64, inner_frac=1), dict(blocks_per_cu=1, tile_w=64, shade_batch=1)):
+            t.tuning(**{k: old[k] for k in ("tile_w", "stack_lds", "blocks_per_cu", "shade_batch", "inner_frac")})
+            t.tuning(**tun)
+            say("tuning", t.tuning())
+            got = _render(rtx, t, 80, 45, 8, 16)
+            for a, b in zip(got, ref):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), tun
+    finally:
+        t.tuning(**{k: old[k] for k in ("tile_w", "stack_lds", "blocks_per_cu", "shade_batch", "inner_frac")})
+
+
 def test_two_level_render_matches_flat_without_ground(rtx, level4_pair):
     """Without the radius-1000 ground sphere the coordinates round at 1e-6
     and almost every path is the same in both layouts."""
