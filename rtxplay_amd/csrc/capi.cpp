@@ -156,6 +156,28 @@ static rtx_status ensure_gstack(rtx::LaunchScratch& ls, uint32_t need) {
     return RTX_SUCCESS;
 }
 
+// the probe cost and tile order arrays of ordered frames (a few hundred KB),
+// allocated by the first launch that needs them and grown with the frame
+static rtx_status ensure_order(rtx::LaunchScratch& ls, uint32_t ntiles) {
+    if (ls.tile_cost && ls.tile_cap >= ntiles) return RTX_SUCCESS;
+    if (ls.tile_cost) {
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipFree(ls.tile_cost));
+        HIP_OK(hipFree(ls.tile_order));
+        ls.tile_cost = ls.tile_order = nullptr;
+        ls.tile_cap = 0;
+    }
+    HIP_OK(hipMalloc((void**)&ls.tile_cost, sizeof(uint32_t) * (size_t)ntiles));
+    hipError_t e = hipMalloc((void**)&ls.tile_order, sizeof(uint32_t) * (size_t)ntiles);
+    if (e != hipSuccess) {
+        (void)hipFree(ls.tile_cost);
+        ls.tile_cost = nullptr;
+        return hip_fail(e, "hipMalloc(tile_order)");
+    }
+    ls.tile_cap = ntiles;
+    return RTX_SUCCESS;
+}
+
 rtx_status rtx_ctx_create(int device, rtx_ctx** out) {
     if (!out) return fail(RTX_ERROR_INVALID_VALUE, "rtx_ctx_create: ctx is NULL");
     int n = 0;
@@ -201,6 +223,8 @@ rtx_status rtx_ctx_destroy(rtx_ctx* c) {
     for (auto* ls : {&c->render_ls, &c->trace_ls}) {
         if (ls->counter) (void)hipFree(ls->counter);
         if (ls->gstack) (void)hipFree(ls->gstack);
+        if (ls->tile_cost) (void)hipFree(ls->tile_cost);
+        if (ls->tile_order) (void)hipFree(ls->tile_order);
     }
     if (c->tape) (void)hipFree(c->tape);
     delete c;
@@ -1011,6 +1035,8 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
         if (lp->spp == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: spp == 0");
         if (lp->depth == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: depth == 0");
         if (!lp->rawRGB) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: rawRGB is NULL");
+        if (lp->spp >= rtx::kOrderMinSpp)
+            if (rtx_status st = ensure_order(c->render_ls, rtx::frame_tiles(*lp, c->render_ls.tile_w))) return st;
         e = rtx::launch_render(b->hdr, *lp, amb, c->render_ls, (hipStream_t)stream);
     }
     if (e != hipSuccess) return hip_fail(e, "rtx_launch");

@@ -24,7 +24,23 @@ struct LaunchScratch {
     uint32_t stack_lds = kStackLDS;   // render: traversal stack entries per lane in LDS (16 or 32)
     uint32_t grid_max = 0;         // blocks the overflow stack area was sized for
     uint32_t gstack_entries = 0;   // overflow entries per lane (beyond the smallest LDS part)
+    uint32_t* tile_cost = nullptr;   // render: rays per tile of the one-sample probe (device)
+    uint32_t* tile_order = nullptr;  // render: the tiles in descending probe cost (device)
+    uint32_t tile_cap = 0;           // tiles the two arrays hold
 };
+
+// Plain frames of at least kOrderMinSpp samples per pixel render their tiles
+// longest first: a one-sample probe launch counts each tile's rays, and the
+// frame takes its tiles in descending order of that count (stable, so equal
+// counts keep their order).  A lane renders whole pixels, so at the end of a
+// frame a wave waits for its most expensive last pixel; longest first makes
+// the last pixels cheap ones.  No pixel depends on the order.
+constexpr uint32_t kOrderMinSpp = 16;
+// tiles of a frame (64 work slots each) for a tile width
+inline uint32_t frame_tiles(const rtx_lp_general& lp, uint32_t tile_w) {
+    const uint32_t th = 64u / tile_w;
+    return ((lp.image_w + tile_w - 1u) / tile_w) * ((lp.image_h + th - 1u) / th);
+}
 
 // One frame of a launch: scene, launch parameters and its slice of the
 // global work index space (64 slots per tile).

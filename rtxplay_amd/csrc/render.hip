@@ -58,6 +58,10 @@ struct RenderArgs {
     uint32_t* tape;           // per-lane path records, word w of slot k at tape[(k*kTapeWords+w)*tape_stride + lane]
     uint32_t tape_stride;     // lanes covered by the tape (grid x block)
     uint32_t lds_grad;        // 1: accumulate per block in LDS first (num_insts x 4 floats)
+    // plain frames: the tile of each tile slot (nullptr: slot order)
+    const uint32_t* tile_order;
+    // probe launch: rays per tile (one sample per pixel, nothing else written)
+    uint32_t* tile_cost;
 };
 
 // path record of the adjoint: meta (instance | type << 24 | flags), the
@@ -88,7 +92,8 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
     return lo;
 }
 
-template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false, bool FMA = true>
+template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false, bool FMA = true,
+          bool PROBE = false>
 __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr) {
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
@@ -101,7 +106,8 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
     // the lane's job: uniform for a single launch, per lane for a batch
     const Job* J = BATCH ? a.jobs : &a.job;
     DevScene sc = a.job.sc;
-    uint32_t W = a.job.lp.image_w, H = a.job.lp.image_h, spp = a.job.lp.spp, maxd = a.job.lp.depth;
+    uint32_t W = a.job.lp.image_w, H = a.job.lp.image_h, spp = PROBE ? 1u : a.job.lp.spp, maxd = a.job.lp.depth;
+    uint32_t tslot = 0;   // probe: tile slot of the lane's pixel
 
     bool active = false, exhausted = false, going = false;
     uint32_t x = 0, y = 0, sample = 0, k = 0, rpp = 0;
@@ -167,7 +173,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                         maxd = J->lp.depth;
                     }
                     const uint32_t li = wi - J->work_begin;
-                    const uint32_t tile = li >> 6, in = li & 63u;
+                    const uint32_t in = li & 63u;
+                    const uint32_t tile = !BATCH && a.tile_order ? a.tile_order[li >> 6] : li >> 6;
+                    if (PROBE) tslot = li >> 6;
                     x = (tile % J->tiles_x) * tw + (in & (tw - 1u));
                     y = (tile / J->tiles_x) * (64u / tw) + (in >> a.tile_log2w);
                     if (x < W && y < H) {
@@ -379,6 +387,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             aacc = aacc + gA;
             if (++sample == spp && ADJ) {
                 active = false;
+            } else if (sample == spp && PROBE) {
+                atomicAdd(&a.tile_cost[tslot], rpp);
+                active = false;
             } else if (sample == spp) {
                 const rtx_lp_general& lp = J->lp;
                 const uint32_t pix = W * y + x;            // optx/camera_r.cu:111-120
@@ -413,6 +424,77 @@ template <int BLOCK, bool COUNT, int SLDS, bool TL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
     render_body<BLOCK, COUNT, false, SLDS, false, TL>(a, s_stack);
+}
+
+// the probe of a plain frame: one sample per pixel, rays counted per tile
+template <int BLOCK, int SLDS, bool TL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_probe(const RenderArgs a) {
+    __shared__ uint32_t s_stack[SLDS * BLOCK];
+    render_body<BLOCK, false, false, SLDS, false, TL, true, true>(a, s_stack);
+}
+
+__global__ void k_zero_n(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// The tiles in descending probe cost, stable: cost quantised to
+// kOrderBuckets classes (class 0 the most rays), a counting sort in one
+// block.  Each chunk of 1024 tiles ranks its tiles within their class by
+// wave ballots (the lanes of a class found bit by bit) and per-wave class
+// counts, so equal classes keep the tiles' order and the result is the same
+// in every run.
+constexpr uint32_t kOrderBuckets = 64;
+constexpr uint32_t kOrderBlock = 1024;
+__global__ __launch_bounds__(kOrderBlock) void k_tile_order(const uint32_t* cost, uint32_t n, uint32_t cmax,
+                                                            uint32_t* order) {
+    __shared__ uint32_t start[kOrderBuckets];                 // running start of each class
+    __shared__ uint32_t wcnt[kOrderBlock / 64][kOrderBuckets];  // per wave: class counts, then offsets
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    auto cls = [&](uint32_t t) -> uint32_t {
+        const uint32_t c = cost[t] < cmax ? cost[t] : cmax;
+        return (kOrderBuckets - 1u) - (uint32_t)(((uint64_t)c * (kOrderBuckets - 1u)) / cmax);
+    };
+    if (tid < kOrderBuckets) start[tid] = 0u;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += kOrderBlock) atomicAdd(&start[cls(t)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k < kOrderBuckets; k++) {
+            const uint32_t c = start[k];
+            start[k] = s;
+            s += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kOrderBlock) {
+        const uint32_t t = base + tid;
+        const bool valid = t < n;
+        const uint32_t b = valid ? cls(t) : kOrderBuckets;   // past the end: a class of its own
+        uint64_t m = ~0ull;
+        for (int bit = 0; bit < 7; bit++) {
+            const uint64_t v = __ballot((b >> bit) & 1u);
+            m &= ((b >> bit) & 1u) ? v : ~v;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        for (uint32_t i = tid; i < (kOrderBlock / 64) * kOrderBuckets; i += kOrderBlock) (&wcnt[0][0])[i] = 0u;
+        __syncthreads();
+        if (valid && (m >> lane) == 1ull) wcnt[wv][b] = rank + 1u;   // the class's highest lane
+        __syncthreads();
+        if (tid < kOrderBuckets) {
+            uint32_t s = start[tid];
+            for (uint32_t w = 0; w < kOrderBlock / 64; w++) {
+                const uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = s;
+                s += c;
+            }
+            start[tid] = s;
+        }
+        __syncthreads();
+        if (valid) order[wcnt[wv][b] + rank] = t;
+        __syncthreads();
+    }
 }
 
 // batch: the same loop with a job per lane, whose job state takes
@@ -568,6 +650,25 @@ hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 am
     a.job = make_job(sc, lp, ambient, 1u << a.tile_log2w, 0);
     a.njobs = 1;
     a.nwork = a.job.work_end;
+    const uint32_t ntiles = a.job.tiles_x * a.job.tiles_y;
+    if (lp.spp >= kOrderMinSpp && ls.tile_cost && ls.tile_cap >= ntiles) {
+        // the probe: one sample per pixel, rays per tile; then the order
+        RenderArgs p = a;
+        p.tile_cost = ls.tile_cost;
+        hipLaunchKernelGGL(k_zero_n, dim3((ntiles + 255u) / 256u), dim3(256), 0, s, ls.tile_cost, ntiles);
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
+        const dim3 g(ls.grid), b(kRenderBlock);
+        if (sc.xinst) {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true>), g, b, 0, s, p);
+        } else {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false>), g, b, 0, s, p);
+        }
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBlock), 0, s, ls.tile_cost, ntiles, 64u * lp.depth,
+                           ls.tile_order);
+        a.tile_order = ls.tile_order;
+    }
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     return launch_frame<false>(a, ls, s);
 }
