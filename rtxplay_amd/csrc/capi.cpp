@@ -1035,7 +1035,7 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
         if (lp->spp == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: spp == 0");
         if (lp->depth == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: depth == 0");
         if (!lp->rawRGB) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch: rawRGB is NULL");
-        if (lp->spp >= rtx::kOrderMinSpp)
+        if (rtx::frame_ordered(*lp, c->render_ls))
             if (rtx_status st = ensure_order(c->render_ls, rtx::frame_tiles(*lp, c->render_ls.tile_w))) return st;
         e = rtx::launch_render(b->hdr, *lp, amb, c->render_ls, (hipStream_t)stream);
     }
@@ -1256,6 +1256,8 @@ rtx_status rtx_launch_adjoint(rtx_ctx* c, const rtx_lp_general* lp, const rtx_fl
         HIP_OK(hipMalloc((void**)&c->tape, words * sizeof(uint32_t)));
         c->tape_depth = lp->depth;
     }
+    if (rtx::frame_ordered(*lp, c->render_ls))
+        if (rtx_status st = ensure_order(c->render_ls, rtx::frame_tiles(*lp, c->render_ls.tile_w))) return st;
     const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
     hipError_t e = rtx::launch_adjoint(b->hdr, *lp, amb, c->render_ls, dL_drgb, d_params, c->tape,
                                        c->render_ls.gstride, (hipStream_t)stream);

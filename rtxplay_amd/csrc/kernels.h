@@ -29,13 +29,20 @@ struct LaunchScratch {
     uint32_t tile_cap = 0;           // tiles the two arrays hold
 };
 
-// Plain frames of at least kOrderMinSpp samples per pixel render their tiles
-// longest first: a one-sample probe launch counts each tile's rays, and the
-// frame takes its tiles in descending order of that count (stable, so equal
-// counts keep their order).  A lane renders whole pixels, so at the end of a
-// frame a wave waits for its most expensive last pixel; longest first makes
-// the last pixels cheap ones.  No pixel depends on the order.
+// Plain frames of at least kOrderMinSpp samples per pixel and at least
+// kOrderMinPixelsPerLane pixels per lane of the persistent grid render their
+// tiles longest first: a one-sample probe launch counts each tile's rays, and
+// the frame takes its tiles in descending order of that count (stable, so
+// equal counts keep their order).  A lane renders whole pixels, so at the end
+// of a frame a wave waits for its most expensive last pixel; longest first
+// makes the last pixels cheap ones.  A frame with fewer pixels than that has
+// no such tail to shorten, and the probe would only add its time.  No pixel
+// depends on the order.
 constexpr uint32_t kOrderMinSpp = 16;
+constexpr uint32_t kOrderMinPixelsPerLane = 2;
+inline bool frame_ordered(const rtx_lp_general& lp, const LaunchScratch& ls) {
+    return lp.spp >= kOrderMinSpp && (uint64_t)lp.image_w * lp.image_h >= (uint64_t)kOrderMinPixelsPerLane * ls.gstride;
+}
 // tiles of a frame (64 work slots each) for a tile width
 inline uint32_t frame_tiles(const rtx_lp_general& lp, uint32_t tile_w) {
     const uint32_t th = 64u / tile_w;

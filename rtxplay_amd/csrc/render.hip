@@ -225,7 +225,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT, TL, FMA>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+        trav_run<BLOCK, COUNT, TL, FMA, BATCH>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
                                wave_iters);
         if (COUNT) cyc_trav += clock64() - c0;
         if (!active || going) continue;
@@ -260,7 +260,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                     const int type = (int)((meta >> 24) & 3u);
                     if (type != RTX_OPTICS_REFRACT) {
                         const float3 P = tget3(j, 1);
-                        const float3 alb = xyz(sc.insts[2 * inst]);
+                        const float3 alb = xyz(ld4<BATCH>(sc.insts + 2 * inst));
                         const float3 gA = delta * P * S;
                         grad(inst, 0, gA.x);
                         grad(inst, 1, gA.y);
@@ -295,14 +295,14 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         } else {
             // hit reconstruction, optx/optics_r.cu:45-90
             const float4* tp = sc.tris + 3 * (size_t)tv.h.tri;
-            const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+            const float4 t0 = ld4<BATCH>(tp), t1 = ld4<BATCH>(tp + 1), t2 = ld4<BATCH>(tp + 2);
             const uint32_t inst = TL ? tv.h.inst : __float_as_uint(t1.w);
-            const float4 m0 = sc.insts[2 * inst], m1 = sc.insts[2 * inst + 1];
+            const float4 m0 = ld4<BATCH>(sc.insts + 2 * inst), m1 = ld4<BATCH>(sc.insts + 2 * inst + 1);
             float3 A = xyz(t0), B = xyz(t1), C = xyz(t2);
             if (TL) {
                 // object-space vertices to world space (optx/optics_r.cu:59-75)
                 const float4* X = sc.xinst + (size_t)kInstF4 * inst;
-                const float4 w0 = X[3], w1 = X[4], w2 = X[5];
+                const float4 w0 = ld4<BATCH>(X + 3), w1 = ld4<BATCH>(X + 4), w2 = ld4<BATCH>(X + 5);
                 A = f3(xf_point(w0, A), xf_point(w1, A), xf_point(w2, A));
                 B = f3(xf_point(w0, B), xf_point(w1, B), xf_point(w2, B));
                 C = f3(xf_point(w0, C), xf_point(w1, C), xf_point(w2, C));
@@ -644,31 +644,42 @@ uint32_t render_blocks_per_cu() {
     return (uint32_t)nb;
 }
 
+namespace {
+// a frame of kOrderMinSpp or more spp (with the scratch in place): the probe
+// launch (one sample per pixel, rays per tile) and the tile order, which the
+// frame's launch then follows (a.tile_order)
+void order_tiles(RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
+    const uint32_t ntiles = a.job.tiles_x * a.job.tiles_y;
+    if (!frame_ordered(a.job.lp, ls) || !ls.tile_cost || ls.tile_cap < ntiles) return;
+    RenderArgs p = a;
+    p.tile_cost = ls.tile_cost;
+    p.tile_order = nullptr;
+    p.dldi = nullptr;
+    p.dparams = nullptr;
+    p.tape = nullptr;
+    hipLaunchKernelGGL(k_zero_n, dim3((ntiles + 255u) / 256u), dim3(256), 0, s, ls.tile_cost, ntiles);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
+    const dim3 g(ls.grid), b(kRenderBlock);
+    if (a.job.sc.xinst) {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true>), g, b, 0, s, p);
+    } else {
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false>), g, b, 0, s, p);
+    }
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBlock), 0, s, ls.tile_cost, ntiles, 64u * a.job.lp.depth,
+                       ls.tile_order);
+    a.tile_order = ls.tile_order;
+}
+}  // namespace
+
 hipError_t launch_render(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,
                          hipStream_t s) {
     RenderArgs a = base_args(ls);
     a.job = make_job(sc, lp, ambient, 1u << a.tile_log2w, 0);
     a.njobs = 1;
     a.nwork = a.job.work_end;
-    const uint32_t ntiles = a.job.tiles_x * a.job.tiles_y;
-    if (lp.spp >= kOrderMinSpp && ls.tile_cost && ls.tile_cap >= ntiles) {
-        // the probe: one sample per pixel, rays per tile; then the order
-        RenderArgs p = a;
-        p.tile_cost = ls.tile_cost;
-        hipLaunchKernelGGL(k_zero_n, dim3((ntiles + 255u) / 256u), dim3(256), 0, s, ls.tile_cost, ntiles);
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
-        const dim3 g(ls.grid), b(kRenderBlock);
-        if (sc.xinst) {
-            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true>), g, b, 0, s, p);
-            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true>), g, b, 0, s, p);
-        } else {
-            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false>), g, b, 0, s, p);
-            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false>), g, b, 0, s, p);
-        }
-        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBlock), 0, s, ls.tile_cost, ntiles, 64u * lp.depth,
-                           ls.tile_order);
-        a.tile_order = ls.tile_order;
-    }
+    order_tiles(a, ls, s);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     return launch_frame<false>(a, ls, s);
 }
@@ -718,6 +729,7 @@ hipError_t launch_adjoint(const DevScene& sc, const rtx_lp_general& lp, float3 a
     a.lds_grad = sc.num_insts <= kAdjointLdsInsts ? 1u : 0u;
     const uint32_t ng = 4u * sc.num_insts;
     if (ng) hipLaunchKernelGGL(k_fill, dim3((ng + 255) / 256), dim3(256), 0, s, dparams, ng, 0.f);
+    order_tiles(a, ls, s);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     const size_t shm = a.lds_grad ? sizeof(float) * ng : 0;
     const dim3 g(ls.grid), b(kRenderBlock);

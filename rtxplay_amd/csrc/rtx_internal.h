@@ -54,6 +54,33 @@ constexpr int kInstF4 = 8;                 // float4 per instance record of a tw
 // wave's traversing lanes wait for one (or when the node phase ends)
 constexpr int kTlFrac = RTX_TL_FRAC;
 
+// The global address space.  A load through such a pointer compiles to a
+// global load.  The batch kernels reach the scene arrays through pointers
+// read from the job table, which the compiler cannot place, so it would load
+// through them with flat loads; those also count against the LDS wait
+// counter, and every LDS stack access would then wait for the node fetches in
+// flight.  The batch kernels load the scene arrays through ld4<true>; the
+// single-frame kernels' arrays come from kernel arguments, which the compiler
+// already places (and the cast costs them 5 % through other code choices).
+#define RTX_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const RTX_GLOBAL T* gbl(const T* p) {
+    return (const RTX_GLOBAL T*)p;
+}
+// one float4 through a global pointer (as a native vector: HIP's float4
+// class cannot be copied from another address space)
+typedef float rtx_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld(const RTX_GLOBAL float4* p) {
+    const rtx_f4v v = *(const RTX_GLOBAL rtx_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+// p[0], through the global address space when G
+template <bool G>
+__device__ __forceinline__ float4 ld4(const float4* p) {
+    if constexpr (G) return gld(gbl(p));
+    else return *p;
+}
+
 // What a traversable handle points at (device memory); kernels get a copy by value.
 struct DevScene {
     const float4* nodes;

@@ -273,7 +273,7 @@ __device__ __forceinline__ float xf_vector(const float4& row, float3 v) {
 // ray and a marker and continues at the instance's bottom-level root with
 // the ray in object space.  The lanes that left, entered or both set their
 // new ray up together, in one call.
-template <class Stack>
+template <bool G, class Stack>
 __device__ __forceinline__ void inst_step(const DevScene& sc, Trav& s, bool going, bool leave, Stack& st) {
     float3 o = s.r.o, d = s.r.d;
     if (leave) {
@@ -288,7 +288,7 @@ __device__ __forceinline__ void inst_step(const DevScene& sc, Trav& s, bool goin
     if (enter) {
         const uint32_t id = s.node & kLeafFirstMask;
         const float4* X = sc.xinst + (size_t)kInstF4 * id;
-        const float4 r0 = X[0], r1 = X[1], r2 = X[2], m = X[6];
+        const float4 r0 = ld4<G>(X), r1 = ld4<G>(X + 1), r2 = ld4<G>(X + 2), m = ld4<G>(X + 6);
         const uint32_t w[7] = {__float_as_uint(o.x), __float_as_uint(o.y), __float_as_uint(o.z), __float_as_uint(d.x),
                                __float_as_uint(d.y), __float_as_uint(d.z), kInstMarker};
         st.push_n(s.sp, w);
@@ -325,7 +325,7 @@ __device__ __forceinline__ bool first_active_lane() {
 // FMA selects the slab form: one FMA per plane with the per-ray offsets
 // (fewer instructions), or (P - o) * idir (six fewer registers, for kernels
 // that are short of them).
-template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, class Stack>
+template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, bool G = false, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
                                          int inner_frac, Stack& st, uint32_t* nn = nullptr,
                                          uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
@@ -344,7 +344,7 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
                 const uint64_t need = __ballot(leave || enter);
                 if (need && (uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * (uint32_t)__popcll(__ballot(going)))
-                    inst_step(sc, s, going, leave, st);
+                    inst_step<G>(sc, s, going, leave, st);
             }
             // a lane that reached a leaf parks it (two slots) and goes on
             // with the next node from its stack, so it keeps stepping while
@@ -370,13 +370,14 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             // axis's entry and exit planes by its direction signs, so the
             // slab test needs no min/max per axis.
             // (32-bit byte offsets: the builder keeps the node array below 4 GiB)
+            typedef const float4 cf4;
             const char* nb = (const char*)sc.nodes;
             const uint32_t no = s.node * (16u * kWideNodeF4);
             const TRay& r = s.r;
-            const float4 ex = *(const float4*)(nb + (no + r.nx)), fx = *(const float4*)(nb + (no + (r.nx ^ 16u)));
-            const float4 ey = *(const float4*)(nb + (no + r.ny)), fy = *(const float4*)(nb + (no + (r.ny ^ 16u)));
-            const float4 ez = *(const float4*)(nb + (no + r.nz)), fz = *(const float4*)(nb + (no + (r.nz ^ 16u)));
-            const float4 rf = *(const float4*)(nb + (no + 96u));
+            const float4 ex = ld4<G>((cf4*)(nb + (no + r.nx))), fx = ld4<G>((cf4*)(nb + (no + (r.nx ^ 16u))));
+            const float4 ey = ld4<G>((cf4*)(nb + (no + r.ny))), fy = ld4<G>((cf4*)(nb + (no + (r.ny ^ 16u))));
+            const float4 ez = ld4<G>((cf4*)(nb + (no + r.nz))), fz = ld4<G>((cf4*)(nb + (no + (r.nz ^ 16u))));
+            const float4 rf = ld4<G>((cf4*)(nb + (no + 96u)));
             float key[4];
             uint32_t ref[4];
             int nhit = 0;
@@ -444,8 +445,9 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 // (32-bit byte offsets: the builder keeps the triangle array below 4 GiB)
                 const char* tb = (const char*)sc.tris;
                 const uint32_t to = k * 48u;
-                const float4 t0 = *(const float4*)(tb + to), t1 = *(const float4*)(tb + (to + 16u)),
-                             t2 = *(const float4*)(tb + (to + 32u));
+                typedef const float4 cf4t;
+                const float4 t0 = ld4<G>((cf4t*)(tb + to)), t1 = ld4<G>((cf4t*)(tb + (to + 16u))),
+                             t2 = ld4<G>((cf4t*)(tb + (to + 32u)));
                 float tt, uu, vv;
                 if (watertight(s.r, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv)) {
                     const uint32_t prim = __float_as_uint(t0.w);
@@ -467,7 +469,7 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             // leaf is parked after the leaf phase)
             const bool leave = going && s.node == kInstMarker;
             const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
-            if (__ballot(leave || enter)) inst_step(sc, s, going, leave, st);
+            if (__ballot(leave || enter)) inst_step<G>(sc, s, going, leave, st);
         }
         if (going && s.node == kNoChild && s.leaf == kNoChild) going = false;
         const uint64_t g = __ballot(going);

@@ -279,6 +279,38 @@ def test_render_odd_sizes_and_repeat(rtx, level4):
     L.free()
 
 
+@pytest.mark.parametrize("tile_w", [8, 4, 32])
+def test_longest_first_order_renders_every_pixel_once(rtx, level4, tile_w):
+    """Frames of 16 or more spp and at least two pixels per lane of the
+    persistent grid (1024 blocks of 256 lanes on 256 CUs) take their tiles
+    in descending order of a one-sample probe's rays; the counted launch
+    keeps the slot order.  On an odd frame size (partial edge tiles) the
+    ordered launch writes every pixel (each buffer is pre-filled with a
+    sentinel) and gives the counted launch's pixels bit for bit."""
+    s, _, _ = level4
+    old = s.tuning()
+    try:
+        s.tuning(tile_w=tile_w)
+        w, h, spp = 1283, 719, 16
+        lp = rtx.LpGeneral()
+        lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, 16, s.handle
+        lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0).c
+        L = rtx.Launcher(s, lp)
+        L.ignite_counted()
+        ref = (L.raw_rgb(), L.rays_per_pixel(), L.normals(), L.albedos())
+        for k in ("rawRGB", "normals", "albedos"):
+            L._bufs[k].upload(np.full((h, w, 3), np.nan, np.float32))
+        L._bufs["rpp"].upload(np.full((h, w), 0xFFFFFFFF, np.uint32))
+        L.ignite()
+        got = (L.raw_rgb(), L.rays_per_pixel(), L.normals(), L.albedos())
+        say("tile_w", tile_w, "ordered launch vs slot order: rays", int(got[1].sum()), int(ref[1].sum()))
+        for a, b in zip(got, ref):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        L.free()
+    finally:
+        s.tuning(**{k: old[k] for k in ("tile_w", "stack_lds", "blocks_per_cu", "shade_batch", "inner_frac")})
+
+
 def test_picker(rtx, orc, level4):
     s, tris, inst = level4
     w, h = 320, 180
