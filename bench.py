@@ -165,12 +165,11 @@ def batched_leg(reps, retune, log_tune):
     tuned = None if retune else autotune.load(TUNED_JSON, ARCH, "batch64")
     source = "stored"
     if tuned is None:
-        # proxy: the same frames at a quarter of the spp (at least 1)
-        proxy = BatchFrames([batch_frame(i) for i in range(64)], spp_scale=0.25)
-        tuned, tsec, hist = autotune.tune_batch(proxy.scenes[0], proxy.lps, log=log_tune)
-        proxy.close()
+        # on the 64 frames themselves: a quarter-spp proxy ranked the shading
+        # batch sizes differently from the full frames
+        tuned, tsec, hist = autotune.tune_batch(ctx, bf.lps, log=log_tune)
         autotune.save(TUNED_JSON, ARCH, "batch64", tuned, tsec,
-                      note="coordinate descent over %d configurations on the 64 frames at 1/4 spp" % len(hist))
+                      note="coordinate descent over %d configurations on the 64 frames" % len(hist))
         source = "autotuned now (%d configurations)" % len(hist)
     ctx.tuning(**tuned)
     b = rtx.Batch(ctx, bf.lps, use_graph=True)

@@ -200,10 +200,10 @@ rtx_status rtx_scene_triangles(rtx_ctx* ctx, uint64_t is_handle, float* tris, ui
 /* Launch tuning of a context (no reference counterpart).  shade_batch: in
  * the render kernel a wave traverses until at least this many of its lanes
  * hold a final closest hit, shades those and resumes the others (1..64; 64 =
- * every lane of the wave finishes first; default 32).  inner_frac: a
+ * every lane of the wave finishes first; default 40).  inner_frac: a
  * traversal round's node phase ends when fewer than inner_frac/64 of the
- * traversing lanes can still step through inner nodes (each lane parks one
- * leaf and goes on); the parked leaves are then tested (1..64; 1 = every lane
+ * traversing lanes can still step through inner nodes (each lane parks up
+ * to two leaves and goes on); the parked leaves are then tested (1..64; 1 = every lane
  * steps as far as it can first, 64 = the first lane that must wait ends the
  * phase; default 36).  tile_w: pixels are handed
  * out in tiles of tile_w x (64 / tile_w) (1, 2, 4, ..., 64; default 8).

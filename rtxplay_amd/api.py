@@ -327,7 +327,7 @@ class Scene:
         return tris, inst
 
     def tuning(self, **kw):
-        """Reads (and with keyword arguments sets) the launch tuning, e.g. shade_batch=32."""
+        """Reads (and with keyword arguments sets) the launch tuning, e.g. shade_batch=40."""
         t = TuningC()
         call("rtx_tuning_get", self.ctx, byref(t))
         if kw:

@@ -19,7 +19,7 @@ import time
 SPACE = {
     "tile_w": (2, 4, 8, 16, 32),
     "stack_lds": (16, 32),
-    "shade_batch": (8, 12, 16, 24, 32),
+    "shade_batch": (8, 12, 16, 24, 32, 40, 48, 56, 64),
     "inner_frac": (24, 30, 36, 42, 48),
 }
 KEYS = ("tile_w", "stack_lds", "blocks_per_cu", "shade_batch", "inner_frac")

@@ -18,7 +18,7 @@ struct LaunchScratch {
     uint32_t* gstack = nullptr;    // traversal stack overflow (device)
     uint32_t gstride = 0;          // lanes covered by gstack (grid * block)
     uint32_t grid = 0;             // persistent grid size in blocks
-    uint32_t shade_batch = 32;     // render: finished lanes that trigger a wave's shading step
+    uint32_t shade_batch = 40;     // render: finished lanes that trigger a wave's shading step
     uint32_t inner_frac = 36;      // traversal: node phase ends below inner_frac/64 of the going lanes
     uint32_t tile_w = 8;           // render: work tile of tile_w x (64 / tile_w) pixels
     uint32_t stack_lds = kStackLDS;   // render: traversal stack entries per lane in LDS (16 or 32)

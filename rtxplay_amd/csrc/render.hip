@@ -614,7 +614,7 @@ RenderArgs base_args(const LaunchScratch& ls) {
     a.counter = ls.counter;
     a.gstack = ls.gstack;
     a.gstride = ls.gstride;
-    a.batch = ls.shade_batch ? ls.shade_batch : 32u;
+    a.batch = ls.shade_batch ? ls.shade_batch : 40u;
     a.inner_frac = ls.inner_frac ? ls.inner_frac : 36u;
     a.tile_log2w = log2u(ls.tile_w ? ls.tile_w : 8u);
     return a;
