@@ -30,6 +30,7 @@ namespace {
 #endif
 constexpr int kBins = RTX_SAH_BINS;
 constexpr int kBuildBlock = 256;
+constexpr uint32_t kSmallTask = 256;   // primitives per task below which a level runs a wave per task
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)
 #ifndef RTX_SAH_TRAV
@@ -186,19 +187,21 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_
 // bins; per-thread runs of equal bins are merged before the LDS atomics),
 // evaluates SAH, chooses leaf / SAH split / median split, partitions stably.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBuildBlock) void k_level(
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_level(
         const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
         const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
         BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
     __shared__ uint32_t s_cnt[3][kBins];
     __shared__ uint32_t s_b[3][kBins][12];   // lo.xyz, hi.xyz, clo.xyz, chi.xyz (ordered ints)
-    __shared__ float s_rarea[3][kBins];
-    __shared__ uint32_t s_rcnt[3][kBins];
+    __shared__ uint32_t s_pcnt[3][kBins];    // primitives in bins 0..b
     __shared__ float s_best_cost[3];
     __shared__ int s_best_bin[3];
+    __shared__ float s_nb[6];                 // node bounds
     __shared__ int s_axis, s_bin, s_kind;
     __shared__ uint32_t s_nleft;
-    __shared__ uint32_t s_ws[kBuildBlock / 64 + 1];
+    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
+    static_assert(kBins == 64, "the SAH evaluation takes one bin per lane of a wave");
 
     const BuildTask task = tasks[blockIdx.x];
     const uint32_t n = task.end - task.begin;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
         scale[k] = ext > 0.f ? (float)kBins * (1.f - 1e-6f) / ext : 0.f;
     }
 
-    for (int i = threadIdx.x; i < 3 * kBins; i += kBuildBlock) {
+    for (int i = threadIdx.x; i < 3 * kBins; i += BLOCK) {
         const int ax = i / kBins, b = i % kBins;
         s_cnt[ax][b] = 0;
         for (int k = 0; k < 12; k++) s_b[ax][b][k] = (k < 3 || (k >= 6 && k < 9)) ? 0xFFFFFFFFu : 0u;
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
 
     // ---- binning: each thread walks a contiguous run of the task's range
     {
-        const uint32_t chunk = (n + kBuildBlock - 1) / kBuildBlock;
+        const uint32_t chunk = (n + BLOCK - 1) / BLOCK;
         const uint32_t b0 = task.begin + min(n, threadIdx.x * chunk);
         const uint32_t b1 = task.begin + min(n, (threadIdx.x + 1) * chunk);
         int cur[3] = {-1, -1, -1};
@@ -266,48 +269,65 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
     }
     __syncthreads();
 
-    // ---- SAH sweep, one thread per axis
-    if (threadIdx.x < 3) {
-        const int ax = threadIdx.x;
-        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-        uint32_t c = 0;
-        for (int b = kBins - 1; b > 0; b--) {
-            if (s_cnt[ax][b]) {
-                lx = fminf(lx, o2f(s_b[ax][b][0])); ly = fminf(ly, o2f(s_b[ax][b][1])); lz = fminf(lz, o2f(s_b[ax][b][2]));
-                hx = fmaxf(hx, o2f(s_b[ax][b][3])); hy = fmaxf(hy, o2f(s_b[ax][b][4])); hz = fmaxf(hz, o2f(s_b[ax][b][5]));
-                c += s_cnt[ax][b];
+    // ---- SAH: the first wave, one lane per bin.  Inclusive prefix (bins
+    // 0..b) and suffix (bins b..63) bounds and counts by shuffles; min and max
+    // are exact, so they equal a bin-by-bin sweep.  The split after bin b
+    // costs area(prefix b) * count + area(suffix b+1) * count; the cheapest
+    // wins, the lowest bin on ties (the order of a sweep with a strict <).
+    if (threadIdx.x < 64) {
+        const int b = (int)threadIdx.x;
+        for (int ax = 0; ax < 3; ax++) {
+            const uint32_t cb = s_cnt[ax][b];
+            float pl[3], ph[3], sl[3], sh[3];
+            for (int k = 0; k < 3; k++) {
+                pl[k] = sl[k] = cb ? o2f(s_b[ax][b][k]) : FLT_MAX;
+                ph[k] = sh[k] = cb ? o2f(s_b[ax][b][3 + k]) : -FLT_MAX;
             }
-            s_rarea[ax][b] = area3(lx, ly, lz, hx, hy, hz);
-            s_rcnt[ax][b] = c;
-        }
-        lx = FLT_MAX; ly = FLT_MAX; lz = FLT_MAX; hx = -FLT_MAX; hy = -FLT_MAX; hz = -FLT_MAX;
-        c = 0;
-        float best = FLT_MAX;
-        int bestb = -1;
-        for (int b = 0; b < kBins - 1; b++) {
-            if (s_cnt[ax][b]) {
-                lx = fminf(lx, o2f(s_b[ax][b][0])); ly = fminf(ly, o2f(s_b[ax][b][1])); lz = fminf(lz, o2f(s_b[ax][b][2]));
-                hx = fmaxf(hx, o2f(s_b[ax][b][3])); hy = fmaxf(hy, o2f(s_b[ax][b][4])); hz = fmaxf(hz, o2f(s_b[ax][b][5]));
-                c += s_cnt[ax][b];
+            uint32_t pc = cb, sc = cb;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const bool up = b >= off, dn = b + off < 64;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float ul = __shfl_up(pl[k], off, 64), uh = __shfl_up(ph[k], off, 64);
+                    const float dl = __shfl_down(sl[k], off, 64), dh = __shfl_down(sh[k], off, 64);
+                    if (up) { pl[k] = fminf(pl[k], ul); ph[k] = fmaxf(ph[k], uh); }
+                    if (dn) { sl[k] = fminf(sl[k], dl); sh[k] = fmaxf(sh[k], dh); }
+                }
+                const uint32_t uc = __shfl_up(pc, off, 64), dc = __shfl_down(sc, off, 64);
+                if (up) pc += uc;
+                if (dn) sc += dc;
             }
-            const uint32_t rc = s_rcnt[ax][b + 1];
-            if (c == 0 || rc == 0) continue;
-            const float cost = area3(lx, ly, lz, hx, hy, hz) * (float)c + s_rarea[ax][b + 1] * (float)rc;
-            if (cost < best) { best = cost; bestb = b; }
+            float rl[3], rh[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) { rl[k] = __shfl_down(sl[k], 1, 64); rh[k] = __shfl_down(sh[k], 1, 64); }
+            const uint32_t rc0 = __shfl_down(sc, 1, 64);
+            const uint32_t rc = b < kBins - 1 ? rc0 : 0u;
+            const float cost = area3(pl[0], pl[1], pl[2], ph[0], ph[1], ph[2]) * (float)pc +
+                               area3(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]) * (float)rc;
+            const bool valid = b < kBins - 1 && pc != 0 && rc != 0 && cost < FLT_MAX;
+            // costs are >= 0, so their bits order like their values
+            uint64_t key = valid ? ((uint64_t)__float_as_uint(cost) << 32) | (uint32_t)b : ~0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(key, off, 64);
+                key = o < key ? o : key;
+            }
+            s_pcnt[ax][b] = pc;
+            if (b == 0) {
+                s_best_cost[ax] = key == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(key >> 32));
+                s_best_bin[ax] = key == ~0ull ? -1 : (int)(uint32_t)key;
+            }
+            if (ax == 0 && b == kBins - 1) {
+                // node bounds: the union of the axis-0 bins
+                for (int k = 0; k < 3; k++) { s_nb[k] = pl[k]; s_nb[3 + k] = ph[k]; }
+            }
         }
-        s_best_cost[ax] = best;
-        s_best_bin[ax] = bestb;
     }
     __syncthreads();
 
     if (threadIdx.x == 0) {
-        // node bounds: union of the axis-0 bins
-        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-        for (int b = 0; b < kBins; b++) {
-            if (!s_cnt[0][b]) continue;
-            lx = fminf(lx, o2f(s_b[0][b][0])); ly = fminf(ly, o2f(s_b[0][b][1])); lz = fminf(lz, o2f(s_b[0][b][2]));
-            hx = fmaxf(hx, o2f(s_b[0][b][3])); hy = fmaxf(hy, o2f(s_b[0][b][4])); hz = fmaxf(hz, o2f(s_b[0][b][5]));
-        }
+        const float lx = s_nb[0], ly = s_nb[1], lz = s_nb[2], hx = s_nb[3], hy = s_nb[4], hz = s_nb[5];
         BNode nd;
         nd.lo = make_float4(lx, ly, lz, 0.f);
         nd.hi = make_float4(hx, hy, hz, 0.f);
@@ -327,13 +347,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
         s_kind = kind;
         s_axis = axis;
         s_bin = axis >= 0 ? s_best_bin[axis] : -1;
-        if (kind == 1) {
-            uint32_t nl = 0;
-            for (int b = 0; b <= s_bin; b++) nl += s_cnt[axis][b];
-            s_nleft = nl;
-        } else {
-            s_nleft = n / 2;
-        }
+        s_nleft = kind == 1 ? s_pcnt[axis][s_bin] : n / 2;
     }
     __syncthreads();
 
@@ -347,7 +361,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
         // stable partition of idx[begin, end) into tmp, then copy back
         const int PER = 4;
         uint32_t lbase = task.begin, rbase = task.begin + s_nleft;
-        for (uint32_t t0 = 0; t0 < n; t0 += kBuildBlock * PER) {
+        for (uint32_t t0 = 0; t0 < n; t0 += BLOCK * PER) {
             uint32_t vals[PER];
             uint32_t flags = 0, nl = 0;
 #pragma unroll
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
                 }
             }
             uint32_t tot_l;
-            const uint32_t pre_l = block_scan<kBuildBlock>(nl, s_ws, tot_l);
+            const uint32_t pre_l = block_scan<BLOCK>(nl, s_ws, tot_l);
             const uint32_t cnt_here = (uint32_t)max(0, min(PER, (int)n - (int)(t0 + threadIdx.x * PER)));
             const uint32_t pre_all = min(n - t0, threadIdx.x * (uint32_t)PER);
             uint32_t l = lbase + pre_l, r = rbase + (pre_all - pre_l);
@@ -372,28 +386,42 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(
                 if ((uint32_t)e >= cnt_here) break;
                 if (flags & (1u << e)) tmp[l++] = vals[e]; else tmp[r++] = vals[e];
             }
-            const uint32_t tile_n = min((uint32_t)(kBuildBlock * PER), n - t0);
+            const uint32_t tile_n = min((uint32_t)(BLOCK * PER), n - t0);
             lbase += tot_l;
             rbase += tile_n - tot_l;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kBuildBlock) idx[task.begin + i] = tmp[task.begin + i];
-        if (threadIdx.x == 0) {
-            float l[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-            float r[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-            for (int b = 0; b < kBins; b++) {
-                if (!s_cnt[ax][b]) continue;
-                float* d = b <= sb ? l : r;
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) idx[task.begin + i] = tmp[task.begin + i];
+        if (threadIdx.x < 64) {
+            // centroid bounds of the two children: bins 0..sb and sb+1..63
+            const int b = (int)threadIdx.x;
+            const bool has = s_cnt[ax][b] != 0, left = b <= sb;
+            float v[12];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float lo = o2f(s_b[ax][b][6 + k]), hi = o2f(s_b[ax][b][9 + k]);
+                v[k] = has && left ? lo : FLT_MAX;
+                v[3 + k] = has && left ? hi : -FLT_MAX;
+                v[6 + k] = has && !left ? lo : FLT_MAX;
+                v[9 + k] = has && !left ? hi : -FLT_MAX;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    d[k] = fminf(d[k], o2f(s_b[ax][b][6 + k]));
-                    d[3 + k] = fmaxf(d[3 + k], o2f(s_b[ax][b][9 + k]));
+                    v[k] = fminf(v[k], __shfl_xor(v[k], off, 64));
+                    v[3 + k] = fmaxf(v[3 + k], __shfl_xor(v[3 + k], off, 64));
+                    v[6 + k] = fminf(v[6 + k], __shfl_xor(v[6 + k], off, 64));
+                    v[9 + k] = fmaxf(v[9 + k], __shfl_xor(v[9 + k], off, 64));
                 }
             }
-            so.clo_l = make_float4(l[0], l[1], l[2], 0.f);
-            so.chi_l = make_float4(l[3], l[4], l[5], 0.f);
-            so.clo_r = make_float4(r[0], r[1], r[2], 0.f);
-            so.chi_r = make_float4(r[3], r[4], r[5], 0.f);
-            out[blockIdx.x] = so;
+            if (b == 0) {
+                so.clo_l = make_float4(v[0], v[1], v[2], 0.f);
+                so.chi_l = make_float4(v[3], v[4], v[5], 0.f);
+                so.clo_r = make_float4(v[6], v[7], v[8], 0.f);
+                so.chi_r = make_float4(v[9], v[10], v[11], 0.f);
+                out[blockIdx.x] = so;
+            }
         }
     } else if (threadIdx.x == 0) {
         // leaf, or median split by index (children inherit the centroid bounds)
@@ -645,8 +673,14 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     BuildTask* cur = tasks_a.p;
     BuildTask* nxt = tasks_b.p;
     while (ntasks > 0) {
-        hipLaunchKernelGGL(k_level, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, tmp.p,
-                           bnodes.p, so.p, max_leaf);
+        // a wave per task once tasks average fewer than kSmallTask primitives
+        // (the same tree: binning, SAH and partition do not depend on the block size)
+        if ((uint64_t)N < (uint64_t)kSmallTask * ntasks)
+            hipLaunchKernelGGL(k_level<64>, dim3(ntasks), dim3(64), 0, s, cur, d_plo, d_phi, idx.p, tmp.p, bnodes.p,
+                               so.p, max_leaf);
+        else
+            hipLaunchKernelGGL(k_level<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p,
+                               tmp.p, bnodes.p, so.p, max_leaf);
         hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, ntasks, rank.p, total.p);
         hipLaunchKernelGGL(k_emit, dim3(blocks_for(ntasks, 256)), dim3(256), 0, s, cur, so.p, rank.p, ntasks,
                            nnodes, bnodes.p, nxt, st.p);
