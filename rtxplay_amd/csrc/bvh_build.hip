@@ -31,6 +31,9 @@ namespace {
 constexpr int kBins = RTX_SAH_BINS;
 constexpr int kBuildBlock = 256;
 constexpr uint32_t kSmallTask = 256;   // primitives per task below which a level runs a wave per task
+constexpr uint32_t kWideTasks = 64;          // levels of at most this many tasks, averaging ...
+constexpr uint32_t kWideMinPrims = 65536;    // ... at least this many primitives, spread each task
+constexpr uint32_t kWideBlocks = 1024;       // over kWideBlocks / tasks chunks
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)
 #ifndef RTX_SAH_TRAV
@@ -183,105 +186,125 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// one BVH level: one block per task.  Bins the task's primitives (3 axes x 32
-// bins; per-thread runs of equal bins are merged before the LDS atomics),
-// evaluates SAH, chooses leaf / SAH split / median split, partitions stably.
+// one BVH level.  Per task: bin the primitives (3 axes x 64 bins; per-thread
+// runs of equal bins are merged before the LDS atomics), evaluate SAH, choose
+// leaf / SAH split / median split, partition stably.  Levels of a few big
+// tasks (the top of the tree) spread each task over many blocks: partial bins
+// per chunk, one block per task merges them and decides, and the partition
+// runs chunk by chunk at offsets from the chunks' left counts.  Binning is a
+// set of min / max / sum and the partition is stable, so both paths give the
+// same tree.
 // ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_level(
-        const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
-        const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
-        BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
-    __shared__ uint32_t s_cnt[3][kBins];
-    __shared__ uint32_t s_b[3][kBins][12];   // lo.xyz, hi.xyz, clo.xyz, chi.xyz (ordered ints)
-    __shared__ uint32_t s_pcnt[3][kBins];    // primitives in bins 0..b
-    __shared__ float s_best_cost[3];
-    __shared__ int s_best_bin[3];
-    __shared__ float s_nb[6];                 // node bounds
-    __shared__ int s_axis, s_bin, s_kind;
-    __shared__ uint32_t s_nleft;
-    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
-    static_assert(kBins == 64, "the SAH evaluation takes one bin per lane of a wave");
+// LDS state of one task's decision
+struct LevelLds {
+    uint32_t cnt[3][kBins];
+    uint32_t b[3][kBins][12];   // lo.xyz, hi.xyz, clo.xyz, chi.xyz (ordered ints)
+    uint32_t pcnt[3][kBins];    // primitives in bins 0..b
+    float best_cost[3];
+    int best_bin[3];
+    float nb[6];                // node bounds
+    int axis, bin, kind;
+    uint32_t nleft;
+};
+static_assert(kBins == 64, "the SAH evaluation takes one bin per lane of a wave");
+constexpr uint32_t kPartialWords = 3 * kBins * 13;   // counts, then bounds, of one chunk's bins
 
-    const BuildTask task = tasks[blockIdx.x];
-    const uint32_t n = task.end - task.begin;
+// a task's binning frame: centroid bounds low corner and bins per unit
+struct TaskGeom {
+    float cl[3], scale[3];
+};
+__device__ __forceinline__ TaskGeom task_geom(const BuildTask& task) {
+    TaskGeom g;
     const float cl[3] = {task.clo.x, task.clo.y, task.clo.z};
     const float ch[3] = {task.chi.x, task.chi.y, task.chi.z};
-    float scale[3];
     for (int k = 0; k < 3; k++) {
         const float ext = ch[k] - cl[k];
-        scale[k] = ext > 0.f ? (float)kBins * (1.f - 1e-6f) / ext : 0.f;
+        g.cl[k] = cl[k];
+        g.scale[k] = ext > 0.f ? (float)kBins * (1.f - 1e-6f) / ext : 0.f;
     }
+    return g;
+}
 
+__device__ __forceinline__ bool bound_is_min(int k) { return k < 3 || (k >= 6 && k < 9); }
+
+template <int BLOCK>
+__device__ __forceinline__ void bins_clear(LevelLds& L) {
     for (int i = threadIdx.x; i < 3 * kBins; i += BLOCK) {
         const int ax = i / kBins, b = i % kBins;
-        s_cnt[ax][b] = 0;
-        for (int k = 0; k < 12; k++) s_b[ax][b][k] = (k < 3 || (k >= 6 && k < 9)) ? 0xFFFFFFFFu : 0u;
+        L.cnt[ax][b] = 0;
+        for (int k = 0; k < 12; k++) L.b[ax][b][k] = bound_is_min(k) ? 0xFFFFFFFFu : 0u;
     }
-    __syncthreads();
+}
 
-    // ---- binning: each thread walks a contiguous run of the task's range
-    {
-        const uint32_t chunk = (n + BLOCK - 1) / BLOCK;
-        const uint32_t b0 = task.begin + min(n, threadIdx.x * chunk);
-        const uint32_t b1 = task.begin + min(n, (threadIdx.x + 1) * chunk);
-        int cur[3] = {-1, -1, -1};
-        uint32_t cnt[3] = {0, 0, 0};
-        uint32_t acc[3][12];
-        auto flush = [&](int ax) {
-            if (cur[ax] < 0 || cnt[ax] == 0) return;
-            const int b = cur[ax];
-            atomicAdd(&s_cnt[ax][b], cnt[ax]);
-            for (int k = 0; k < 3; k++) {
-                atomicMin(&s_b[ax][b][k], acc[ax][k]);
-                atomicMax(&s_b[ax][b][3 + k], acc[ax][3 + k]);
-                atomicMin(&s_b[ax][b][6 + k], acc[ax][6 + k]);
-                atomicMax(&s_b[ax][b][9 + k], acc[ax][9 + k]);
+// bins idx[r0, r1): each thread walks a contiguous run of the range
+template <int BLOCK>
+__device__ __forceinline__ void bins_add(LevelLds& L, const TaskGeom& g, const uint32_t* __restrict__ idx,
+                                         const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                         uint32_t r0, uint32_t r1) {
+    const uint32_t n = r1 - r0;
+    const uint32_t chunk = (n + BLOCK - 1) / BLOCK;
+    const uint32_t b0 = r0 + min(n, threadIdx.x * chunk);
+    const uint32_t b1 = r0 + min(n, (threadIdx.x + 1) * chunk);
+    int cur[3] = {-1, -1, -1};
+    uint32_t cnt[3] = {0, 0, 0};
+    uint32_t acc[3][12];
+    auto flush = [&](int ax) {
+        if (cur[ax] < 0 || cnt[ax] == 0) return;
+        const int b = cur[ax];
+        atomicAdd(&L.cnt[ax][b], cnt[ax]);
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&L.b[ax][b][k], acc[ax][k]);
+            atomicMax(&L.b[ax][b][3 + k], acc[ax][3 + k]);
+            atomicMin(&L.b[ax][b][6 + k], acc[ax][6 + k]);
+            atomicMax(&L.b[ax][b][9 + k], acc[ax][9 + k]);
+        }
+    };
+    for (uint32_t i = b0; i < b1; i++) {
+        const uint32_t p = idx[i];
+        const float4 a = plo[p], h = phi[p];
+        const float c[3] = {.5f * (a.x + h.x), .5f * (a.y + h.y), .5f * (a.z + h.z)};
+        const uint32_t v[12] = {f2o(a.x), f2o(a.y), f2o(a.z), f2o(h.x), f2o(h.y), f2o(h.z),
+                                f2o(c[0]), f2o(c[1]), f2o(c[2]), f2o(c[0]), f2o(c[1]), f2o(c[2])};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const int b = bin_of(c[ax], g.cl[ax], g.scale[ax]);
+            if (b != cur[ax]) {
+                flush(ax);
+                cur[ax] = b;
+                cnt[ax] = 0;
+#pragma unroll
+                for (int k = 0; k < 12; k++) acc[ax][k] = v[k];
             }
-        };
-        for (uint32_t i = b0; i < b1; i++) {
-            const uint32_t p = idx[i];
-            const float4 a = plo[p], h = phi[p];
-            const float c[3] = {.5f * (a.x + h.x), .5f * (a.y + h.y), .5f * (a.z + h.z)};
-            const uint32_t v[12] = {f2o(a.x), f2o(a.y), f2o(a.z), f2o(h.x), f2o(h.y), f2o(h.z),
-                                    f2o(c[0]), f2o(c[1]), f2o(c[2]), f2o(c[0]), f2o(c[1]), f2o(c[2])};
+            cnt[ax]++;
 #pragma unroll
-            for (int ax = 0; ax < 3; ax++) {
-                const int b = bin_of(c[ax], cl[ax], scale[ax]);
-                if (b != cur[ax]) {
-                    flush(ax);
-                    cur[ax] = b;
-                    cnt[ax] = 0;
-#pragma unroll
-                    for (int k = 0; k < 12; k++) acc[ax][k] = v[k];
-                }
-                cnt[ax]++;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    acc[ax][k] = min(acc[ax][k], v[k]);
-                    acc[ax][3 + k] = max(acc[ax][3 + k], v[3 + k]);
-                    acc[ax][6 + k] = min(acc[ax][6 + k], v[6 + k]);
-                    acc[ax][9 + k] = max(acc[ax][9 + k], v[9 + k]);
-                }
+            for (int k = 0; k < 3; k++) {
+                acc[ax][k] = min(acc[ax][k], v[k]);
+                acc[ax][3 + k] = max(acc[ax][3 + k], v[3 + k]);
+                acc[ax][6 + k] = min(acc[ax][6 + k], v[6 + k]);
+                acc[ax][9 + k] = max(acc[ax][9 + k], v[9 + k]);
             }
         }
-        flush(0); flush(1); flush(2);
     }
-    __syncthreads();
+    flush(0); flush(1); flush(2);
+}
 
-    // ---- SAH: the first wave, one lane per bin.  Inclusive prefix (bins
-    // 0..b) and suffix (bins b..63) bounds and counts by shuffles; min and max
-    // are exact, so they equal a bin-by-bin sweep.  The split after bin b
-    // costs area(prefix b) * count + area(suffix b+1) * count; the cheapest
-    // wins, the lowest bin on ties (the order of a sweep with a strict <).
+// SAH decision from the task's bins (every thread of the block calls it):
+// the first wave evaluates the splits, one lane per bin -- inclusive prefix
+// (bins 0..b) and suffix (bins b..63) bounds and counts by shuffles; min and
+// max are exact, so they equal a bin-by-bin sweep.  The split after bin b
+// costs area(prefix b) * count + area(suffix b+1) * count; the cheapest wins,
+// the lowest bin on ties (the order of a sweep with a strict <).  Thread 0
+// then writes the node's bounds and chooses leaf / SAH split / median split.
+__device__ __forceinline__ void sah_decide(LevelLds& L, const BuildTask& task, uint32_t n,
+                                           BNode* __restrict__ bnodes, uint32_t max_leaf) {
     if (threadIdx.x < 64) {
         const int b = (int)threadIdx.x;
         for (int ax = 0; ax < 3; ax++) {
-            const uint32_t cb = s_cnt[ax][b];
+            const uint32_t cb = L.cnt[ax][b];
             float pl[3], ph[3], sl[3], sh[3];
             for (int k = 0; k < 3; k++) {
-                pl[k] = sl[k] = cb ? o2f(s_b[ax][b][k]) : FLT_MAX;
-                ph[k] = sh[k] = cb ? o2f(s_b[ax][b][3 + k]) : -FLT_MAX;
+                pl[k] = sl[k] = cb ? o2f(L.b[ax][b][k]) : FLT_MAX;
+                ph[k] = sh[k] = cb ? o2f(L.b[ax][b][3 + k]) : -FLT_MAX;
             }
             uint32_t pc = cb, sc = cb;
 #pragma unroll
@@ -313,21 +336,20 @@ __global__ __launch_bounds__(BLOCK) void k_level(
                 const uint64_t o = __shfl_xor(key, off, 64);
                 key = o < key ? o : key;
             }
-            s_pcnt[ax][b] = pc;
+            L.pcnt[ax][b] = pc;
             if (b == 0) {
-                s_best_cost[ax] = key == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(key >> 32));
-                s_best_bin[ax] = key == ~0ull ? -1 : (int)(uint32_t)key;
+                L.best_cost[ax] = key == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(key >> 32));
+                L.best_bin[ax] = key == ~0ull ? -1 : (int)(uint32_t)key;
             }
             if (ax == 0 && b == kBins - 1) {
                 // node bounds: the union of the axis-0 bins
-                for (int k = 0; k < 3; k++) { s_nb[k] = pl[k]; s_nb[3 + k] = ph[k]; }
+                for (int k = 0; k < 3; k++) { L.nb[k] = pl[k]; L.nb[3 + k] = ph[k]; }
             }
         }
     }
     __syncthreads();
-
     if (threadIdx.x == 0) {
-        const float lx = s_nb[0], ly = s_nb[1], lz = s_nb[2], hx = s_nb[3], hy = s_nb[4], hz = s_nb[5];
+        const float lx = L.nb[0], ly = L.nb[1], lz = L.nb[2], hx = L.nb[3], hy = L.nb[4], hz = L.nb[5];
         BNode nd;
         nd.lo = make_float4(lx, ly, lz, 0.f);
         nd.hi = make_float4(hx, hy, hz, 0.f);
@@ -335,7 +357,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(
         int axis = -1;
         float best = FLT_MAX;
         for (int ax = 0; ax < 3; ax++)
-            if (s_best_bin[ax] >= 0 && s_best_cost[ax] < best) { best = s_best_cost[ax]; axis = ax; }
+            if (L.best_bin[ax] >= 0 && L.best_cost[ax] < best) { best = L.best_cost[ax]; axis = ax; }
         const float A = area3(lx, ly, lz, hx, hy, hz);
         const float leaf_cost = kCostIsect * (float)n * A;
         const float split_cost = kCostTrav * A + kCostIsect * best;
@@ -344,62 +366,31 @@ __global__ __launch_bounds__(BLOCK) void k_level(
         else if (n <= max_leaf && (axis < 0 || leaf_cost <= split_cost)) kind = 0;
         else if (axis < 0 || task.depth >= (uint32_t)(kMaxTreeDepth - 28)) kind = 2;
         else kind = 1;
-        s_kind = kind;
-        s_axis = axis;
-        s_bin = axis >= 0 ? s_best_bin[axis] : -1;
-        s_nleft = kind == 1 ? s_pcnt[axis][s_bin] : n / 2;
+        L.kind = kind;
+        L.axis = axis;
+        L.bin = axis >= 0 ? L.best_bin[axis] : -1;
+        L.nleft = kind == 1 ? L.pcnt[axis][L.bin] : n / 2;
     }
     __syncthreads();
+}
 
-    const int kind = s_kind;
+// the task's split record: for a SAH split the centroid bounds of the two
+// children (bins 0..bin and bin+1..63, reduced by the first wave); for a
+// leaf or a median split by index the task's own bounds
+__device__ __forceinline__ void split_out(const LevelLds& L, const BuildTask& task, SplitOut* __restrict__ out,
+                                          uint32_t slot) {
     SplitOut so;
-    so.kind = kind == 0 ? 0u : 1u;
-    so.nleft = s_nleft;
+    so.kind = L.kind == 0 ? 0u : 1u;
+    so.nleft = L.nleft;
     so.pad0 = so.pad1 = 0;
-    if (kind == 1) {
-        const int ax = s_axis, sb = s_bin;
-        // stable partition of idx[begin, end) into tmp, then copy back
-        const int PER = 4;
-        uint32_t lbase = task.begin, rbase = task.begin + s_nleft;
-        for (uint32_t t0 = 0; t0 < n; t0 += BLOCK * PER) {
-            uint32_t vals[PER];
-            uint32_t flags = 0, nl = 0;
-#pragma unroll
-            for (int e = 0; e < PER; e++) {
-                const uint32_t i = t0 + threadIdx.x * PER + e;
-                vals[e] = 0;
-                if (i < n) {
-                    const uint32_t p = idx[task.begin + i];
-                    vals[e] = p;
-                    const float4 a = plo[p], h = phi[p];
-                    const float c = .5f * (comp4(a, ax) + comp4(h, ax));
-                    if (bin_of(c, cl[ax], scale[ax]) <= sb) { flags |= 1u << e; nl++; }
-                }
-            }
-            uint32_t tot_l;
-            const uint32_t pre_l = block_scan<BLOCK>(nl, s_ws, tot_l);
-            const uint32_t cnt_here = (uint32_t)max(0, min(PER, (int)n - (int)(t0 + threadIdx.x * PER)));
-            const uint32_t pre_all = min(n - t0, threadIdx.x * (uint32_t)PER);
-            uint32_t l = lbase + pre_l, r = rbase + (pre_all - pre_l);
-#pragma unroll
-            for (int e = 0; e < PER; e++) {
-                if ((uint32_t)e >= cnt_here) break;
-                if (flags & (1u << e)) tmp[l++] = vals[e]; else tmp[r++] = vals[e];
-            }
-            const uint32_t tile_n = min((uint32_t)(BLOCK * PER), n - t0);
-            lbase += tot_l;
-            rbase += tile_n - tot_l;
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) idx[task.begin + i] = tmp[task.begin + i];
+    if (L.kind == 1) {
         if (threadIdx.x < 64) {
-            // centroid bounds of the two children: bins 0..sb and sb+1..63
-            const int b = (int)threadIdx.x;
-            const bool has = s_cnt[ax][b] != 0, left = b <= sb;
+            const int ax = L.axis, sb = L.bin, b = (int)threadIdx.x;
+            const bool has = L.cnt[ax][b] != 0, left = b <= sb;
             float v[12];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float lo = o2f(s_b[ax][b][6 + k]), hi = o2f(s_b[ax][b][9 + k]);
+                const float lo = o2f(L.b[ax][b][6 + k]), hi = o2f(L.b[ax][b][9 + k]);
                 v[k] = has && left ? lo : FLT_MAX;
                 v[3 + k] = has && left ? hi : -FLT_MAX;
                 v[6 + k] = has && !left ? lo : FLT_MAX;
@@ -420,15 +411,201 @@ __global__ __launch_bounds__(BLOCK) void k_level(
                 so.chi_l = make_float4(v[3], v[4], v[5], 0.f);
                 so.clo_r = make_float4(v[6], v[7], v[8], 0.f);
                 so.chi_r = make_float4(v[9], v[10], v[11], 0.f);
-                out[blockIdx.x] = so;
+                out[slot] = so;
             }
         }
     } else if (threadIdx.x == 0) {
-        // leaf, or median split by index (children inherit the centroid bounds)
         so.clo_l = so.clo_r = task.clo;
         so.chi_l = so.chi_r = task.chi;
-        out[blockIdx.x] = so;
+        out[slot] = so;
     }
+}
+
+// stable partition of idx[r0, r1) into tmp: the range's primitives left of
+// the split go to tmp[lbase ...], the others to tmp[rbase ...], each in order
+template <int BLOCK>
+__device__ __forceinline__ void partition_range(const TaskGeom& g, int ax, int sb, const uint32_t* __restrict__ idx,
+                                                uint32_t* __restrict__ tmp, const float4* __restrict__ plo,
+                                                const float4* __restrict__ phi, uint32_t r0, uint32_t r1,
+                                                uint32_t lbase, uint32_t rbase, uint32_t* ws) {
+    const int PER = 4;
+    const uint32_t n = r1 - r0;
+    for (uint32_t t0 = 0; t0 < n; t0 += BLOCK * PER) {
+        uint32_t vals[PER];
+        uint32_t flags = 0, nl = 0;
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const uint32_t i = t0 + threadIdx.x * PER + e;
+            vals[e] = 0;
+            if (i < n) {
+                const uint32_t p = idx[r0 + i];
+                vals[e] = p;
+                const float4 a = plo[p], h = phi[p];
+                const float c = .5f * (comp4(a, ax) + comp4(h, ax));
+                if (bin_of(c, g.cl[ax], g.scale[ax]) <= sb) { flags |= 1u << e; nl++; }
+            }
+        }
+        uint32_t tot_l;
+        const uint32_t pre_l = block_scan<BLOCK>(nl, ws, tot_l);
+        const uint32_t cnt_here = (uint32_t)max(0, min(PER, (int)n - (int)(t0 + threadIdx.x * PER)));
+        const uint32_t pre_all = min(n - t0, threadIdx.x * (uint32_t)PER);
+        uint32_t l = lbase + pre_l, r = rbase + (pre_all - pre_l);
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            if ((uint32_t)e >= cnt_here) break;
+            if (flags & (1u << e)) tmp[l++] = vals[e]; else tmp[r++] = vals[e];
+        }
+        const uint32_t tile_n = min((uint32_t)(BLOCK * PER), n - t0);
+        lbase += tot_l;
+        rbase += tile_n - tot_l;
+    }
+}
+
+// one block per task
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_level(
+        const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+        const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+        BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
+    __shared__ LevelLds L;
+    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
+    const BuildTask task = tasks[blockIdx.x];
+    const uint32_t n = task.end - task.begin;
+    const TaskGeom g = task_geom(task);
+    bins_clear<BLOCK>(L);
+    __syncthreads();
+    bins_add<BLOCK>(L, g, idx, plo, phi, task.begin, task.end);
+    __syncthreads();
+    sah_decide(L, task, n, bnodes, max_leaf);
+    if (L.kind == 1) {
+        partition_range<BLOCK>(g, L.axis, L.bin, idx, tmp, plo, phi, task.begin, task.end, task.begin,
+                               task.begin + L.nleft, s_ws);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) idx[task.begin + i] = tmp[task.begin + i];
+    }
+    split_out(L, task, out, blockIdx.x);
+}
+
+// big tasks, step 1: block (chunk c, task t) bins chunk c of the task into
+// partial[t * G + c]
+__device__ __forceinline__ void chunk_range(const BuildTask& task, uint32_t G, uint32_t c, uint32_t& r0,
+                                            uint32_t& r1) {
+    const uint32_t n = task.end - task.begin, chunk = (n + G - 1) / G;
+    r0 = task.begin + min(n, c * chunk);
+    r1 = task.begin + min(n, (c + 1) * chunk);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bin_wide(const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+                                                    const float4* __restrict__ phi, const uint32_t* __restrict__ idx,
+                                                    uint32_t G, uint32_t* __restrict__ partial) {
+    __shared__ LevelLds L;
+    const BuildTask task = tasks[blockIdx.y];
+    uint32_t r0, r1;
+    chunk_range(task, G, blockIdx.x, r0, r1);
+    bins_clear<BLOCK>(L);
+    __syncthreads();
+    bins_add<BLOCK>(L, task_geom(task), idx, plo, phi, r0, r1);
+    __syncthreads();
+    uint32_t* P = partial + (size_t)(blockIdx.y * G + blockIdx.x) * kPartialWords;
+    const uint32_t* c = &L.cnt[0][0];
+    const uint32_t* bb = &L.b[0][0][0];
+    for (uint32_t i = threadIdx.x; i < 3 * kBins; i += BLOCK) P[i] = c[i];
+    for (uint32_t i = threadIdx.x; i < 3 * kBins * 12; i += BLOCK) P[3 * kBins + i] = bb[i];
+}
+
+// big tasks, step 2: one block per task merges the chunks' bins and decides
+// as k_level does; wsplit[t] = (kind, axis, bin, nleft) for the partition
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_decide_wide(const BuildTask* __restrict__ tasks, uint32_t G,
+                                                       const uint32_t* __restrict__ partial,
+                                                       BNode* __restrict__ bnodes, SplitOut* __restrict__ out,
+                                                       uint4* __restrict__ wsplit, uint32_t max_leaf) {
+    __shared__ LevelLds L;
+    const BuildTask task = tasks[blockIdx.x];
+    const uint32_t* P = partial + (size_t)blockIdx.x * G * kPartialWords;
+    uint32_t* c = &L.cnt[0][0];
+    uint32_t* bb = &L.b[0][0][0];
+    for (uint32_t i = threadIdx.x; i < 3 * kBins; i += BLOCK) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k < G; k++) s += P[(size_t)k * kPartialWords + i];
+        c[i] = s;
+    }
+    for (uint32_t i = threadIdx.x; i < 3 * kBins * 12; i += BLOCK) {
+        const bool mn = bound_is_min((int)(i % 12));
+        uint32_t v = mn ? 0xFFFFFFFFu : 0u;
+        for (uint32_t k = 0; k < G; k++) {
+            const uint32_t x = P[(size_t)k * kPartialWords + 3 * kBins + i];
+            v = mn ? min(v, x) : max(v, x);
+        }
+        bb[i] = v;
+    }
+    __syncthreads();
+    sah_decide(L, task, task.end - task.begin, bnodes, max_leaf);
+    split_out(L, task, out, blockIdx.x);
+    if (threadIdx.x == 0) wsplit[blockIdx.x] = make_uint4((uint32_t)L.kind, (uint32_t)L.axis, (uint32_t)L.bin, L.nleft);
+}
+
+// big tasks, step 3: primitives left of the split in each chunk
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_count_wide(const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+                                                      const float4* __restrict__ phi, const uint32_t* __restrict__ idx,
+                                                      uint32_t G, const uint4* __restrict__ wsplit,
+                                                      uint32_t* __restrict__ lcount) {
+    __shared__ uint32_t s_n;
+    const uint4 w = wsplit[blockIdx.y];
+    if (w.x != 1u) return;   // the whole block: no SAH split
+    const BuildTask task = tasks[blockIdx.y];
+    const TaskGeom g = task_geom(task);
+    uint32_t r0, r1;
+    chunk_range(task, G, blockIdx.x, r0, r1);
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int ax = (int)w.y, sb = (int)w.z;
+    uint32_t nl = 0;
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const uint32_t p = idx[i];
+        const float4 a = plo[p], h = phi[p];
+        if (bin_of(.5f * (comp4(a, ax) + comp4(h, ax)), g.cl[ax], g.scale[ax]) <= sb) nl++;
+    }
+    atomicAdd(&s_n, nl);
+    __syncthreads();
+    if (threadIdx.x == 0) lcount[blockIdx.y * G + blockIdx.x] = s_n;
+}
+
+// big tasks, step 4: each chunk partitions stably into tmp after the left
+// and right primitives of the chunks before it
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_scatter_wide(const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+                                                        const float4* __restrict__ phi, const uint32_t* __restrict__ idx,
+                                                        uint32_t* __restrict__ tmp, uint32_t G,
+                                                        const uint4* __restrict__ wsplit,
+                                                        const uint32_t* __restrict__ lcount) {
+    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
+    __shared__ uint32_t s_lb;
+    const uint4 w = wsplit[blockIdx.y];
+    if (w.x != 1u) return;
+    const BuildTask task = tasks[blockIdx.y];
+    uint32_t r0, r1;
+    chunk_range(task, G, blockIdx.x, r0, r1);
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k < blockIdx.x; k++) s += lcount[blockIdx.y * G + k];
+        s_lb = s;
+    }
+    __syncthreads();
+    const uint32_t lb = s_lb, before = r0 - task.begin;
+    partition_range<BLOCK>(task_geom(task), (int)w.y, (int)w.z, idx, tmp, plo, phi, r0, r1, task.begin + lb,
+                           task.begin + w.w + (before - lb), s_ws);
+}
+
+// big tasks, step 5: the partitioned ranges back into idx
+__global__ void k_copy_wide(const BuildTask* __restrict__ tasks, uint32_t G, const uint4* __restrict__ wsplit,
+                            const uint32_t* __restrict__ tmp, uint32_t* __restrict__ idx) {
+    if (wsplit[blockIdx.y].x != 1u) return;
+    uint32_t r0, r1;
+    chunk_range(tasks[blockIdx.y], G, blockIdx.x, r0, r1);
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) idx[i] = tmp[i];
 }
 
 // single-block exclusive scan of `n` uints (kind flags) -> out, total -> *total
@@ -664,6 +841,13 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     RTX_TRY(bnodes.alloc(max_nodes));
     RTX_TRY(st.alloc(1));
     RTX_TRY(hipMemsetAsync(st.p, 0, sizeof(BuildStats), s));
+    DevBuf<uint32_t> partial, lcount;
+    DevBuf<uint4> wsplit;
+    if (N >= kWideMinPrims) {
+        RTX_TRY(partial.alloc((size_t)kWideBlocks * kPartialWords));
+        RTX_TRY(lcount.alloc(kWideBlocks));
+        RTX_TRY(wsplit.alloc(kWideTasks));
+    }
 
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(N, 256)), dim3(256), 0, s, idx.p, N);
     hipLaunchKernelGGL(k_root_task, dim3(1), dim3(1024), 0, s, d_plo, d_phi, N, tasks_a.p, idx.p);
@@ -673,9 +857,22 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     BuildTask* cur = tasks_a.p;
     BuildTask* nxt = tasks_b.p;
     while (ntasks > 0) {
-        // a wave per task once tasks average fewer than kSmallTask primitives
-        // (the same tree: binning, SAH and partition do not depend on the block size)
-        if ((uint64_t)N < (uint64_t)kSmallTask * ntasks)
+        // a few big tasks: each spread over kWideBlocks / ntasks chunks; a wave
+        // per task once tasks average fewer than kSmallTask primitives (the same
+        // tree either way: binning, SAH and partition do not depend on the split)
+        if (ntasks <= kWideTasks && (uint64_t)N >= (uint64_t)kWideMinPrims * ntasks) {
+            const uint32_t G = kWideBlocks / ntasks;
+            const dim3 gw(G, ntasks);
+            hipLaunchKernelGGL(k_bin_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, G,
+                               partial.p);
+            hipLaunchKernelGGL(k_decide_wide<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, G, partial.p,
+                               bnodes.p, so.p, wsplit.p, max_leaf);
+            hipLaunchKernelGGL(k_count_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, G,
+                               wsplit.p, lcount.p);
+            hipLaunchKernelGGL(k_scatter_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p,
+                               tmp.p, G, wsplit.p, lcount.p);
+            hipLaunchKernelGGL(k_copy_wide, gw, dim3(256), 0, s, cur, G, wsplit.p, tmp.p, idx.p);
+        } else if ((uint64_t)N < (uint64_t)kSmallTask * ntasks)
             hipLaunchKernelGGL(k_level<64>, dim3(ntasks), dim3(64), 0, s, cur, d_plo, d_phi, idx.p, tmp.p, bnodes.p,
                                so.p, max_leaf);
         else
