@@ -624,6 +624,37 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_kind(const SplitOut* __rest
     if (threadIdx.x == 0) *total = carry;
 }
 
+// the same scan over many blocks, for long arrays: tile sums, a scan of
+// them (k_scan_u32), then each tile rescanned from its offset
+constexpr uint32_t kScanTile = 4 * kScanBlock;
+__global__ __launch_bounds__(kScanBlock) void k_kind_tiles(const SplitOut* __restrict__ so, uint32_t n,
+                                                          uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    const uint32_t b0 = blockIdx.x * kScanTile;
+    for (uint32_t i = b0 + threadIdx.x; i < n && i < b0 + kScanTile; i += kScanBlock) c += so[i].kind;
+    atomicAdd(&s_n, c);
+    __syncthreads();
+    if (threadIdx.x == 0) tsum[blockIdx.x] = s_n;
+}
+__global__ __launch_bounds__(kScanBlock) void k_kind_apply(const SplitOut* __restrict__ so, uint32_t n,
+                                                          const uint32_t* __restrict__ toff,
+                                                          uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_ws[kScanBlock / 64 + 1];
+    uint32_t carry = toff[blockIdx.x];
+    const uint32_t b0 = blockIdx.x * kScanTile;
+    for (uint32_t base = b0; base < b0 + kScanTile && base < n; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? so[i].kind : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan<kScanBlock>(v, s_ws, tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+}
+
 __global__ void k_emit(const BuildTask* __restrict__ tasks, const SplitOut* __restrict__ so,
                        const uint32_t* __restrict__ rank, uint32_t ntasks, uint32_t node_base,
                        BNode* __restrict__ bnodes, BuildTask* __restrict__ next,
@@ -853,6 +884,21 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     hipLaunchKernelGGL(k_root_task, dim3(1), dim3(1024), 0, s, d_plo, d_phi, N, tasks_a.p, idx.p);
     RTX_TRY(hipGetLastError());
 
+    // exclusive scan of so[0, n).kind into rank, the sum into total
+    DevBuf<uint32_t> tsum, toff;
+    RTX_TRY(tsum.alloc((size_t)max_nodes / kScanTile + 2));
+    RTX_TRY(toff.alloc((size_t)max_nodes / kScanTile + 2));
+    auto scan_kind = [&](uint32_t n) {
+        if (n <= 16 * kScanTile) {
+            hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, n, rank.p, total.p);
+            return;
+        }
+        const uint32_t nt = (n + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(k_kind_tiles, dim3(nt), dim3(kScanBlock), 0, s, so.p, n, tsum.p);
+        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, tsum.p, nt, toff.p, total.p);
+        hipLaunchKernelGGL(k_kind_apply, dim3(nt), dim3(kScanBlock), 0, s, so.p, n, toff.p, rank.p);
+    };
+
     uint32_t ntasks = 1, nnodes = 1;
     BuildTask* cur = tasks_a.p;
     BuildTask* nxt = tasks_b.p;
@@ -878,7 +924,7 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         else
             hipLaunchKernelGGL(k_level<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p,
                                tmp.p, bnodes.p, so.p, max_leaf);
-        hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, ntasks, rank.p, total.p);
+        scan_kind(ntasks);
         hipLaunchKernelGGL(k_emit, dim3(blocks_for(ntasks, 256)), dim3(256), 0, s, cur, so.p, rank.p, ntasks,
                            nnodes, bnodes.p, nxt, st.p);
         RTX_TRY(hipGetLastError());
@@ -893,7 +939,7 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     hipLaunchKernelGGL(k_sah, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, st.p);
     // compact inner nodes into the traversal layout (BFS order of the build)
     hipLaunchKernelGGL(k_inner_flag, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, so.p);
-    hipLaunchKernelGGL(k_scan_kind, dim3(1), dim3(kScanBlock), 0, s, so.p, nnodes, rank.p, total.p);
+    scan_kind(nnodes);
     RTX_TRY(hipGetLastError());
     uint32_t ninner = 0;
     BuildStats hst;
