@@ -89,6 +89,18 @@ def test_build_level4_stats(level4):
     assert st["max_depth"] <= 64 and st["num_nodes"] == st["num_leaves"] - 1
 
 
+def test_build_tree_is_unchanged(level4):
+    """The builder's tree for the level-4 scene, pinned: the block-per-task,
+    wave-per-task and chunked (wide) level paths must all give this tree
+    (binning is sums, minima and maxima, and the partition is stable), so a
+    change of any of them that alters the tree fails here.  The values are
+    the builder's own output, recorded when the paths were introduced."""
+    s, _, _ = level4
+    st = s.stats()
+    assert (st["num_nodes"], st["num_leaves"], st["max_depth"]) == (291545, 291546, 27)
+    assert abs(st["sah_cost"] - 27.92353057861328) < 1e-9
+
+
 def test_build_is_deterministic(rtx, small):
     s, tris, _ = small
     a = s.trace(_rays(4096, 11))
