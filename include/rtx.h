@@ -231,7 +231,11 @@ rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
 /* Launcher::ignite, optx/launcher.cxx:49-77 (optixLaunch of the raygen program
  * optx/camera_r.cu:23-121 with the closest-hit programs optx/optics_r.cu).
  * Asynchronous on `stream` (the C++ Launcher syncs, as the reference does).
- * once != 0 launches the 1x1 picker path (lp->picker). */
+ * once != 0 launches the 1x1 picker path (lp->picker).  A frame of 16 or more
+ * spp and at least two pixels per lane of the persistent grid first runs a
+ * one-sample probe and renders its 64-pixel tiles in descending order of the
+ * probe's rays (the pixels do not depend on the order); the first such launch
+ * of a context allocates the order's arrays (8 bytes per tile). */
 rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
 
 /* Batched render (no reference counterpart; SURVEY config 4): n frames, each
