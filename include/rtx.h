@@ -276,7 +276,8 @@ rtx_status rtx_scene_get_materials(rtx_ctx* ctx, uint64_t is_handle, float* para
  * hold the discrete path structure fixed (hit triangles, reflect/refract and
  * absorb decisions), the interior derivative; pixels whose mean clamps to 0
  * or 1 pass no gradient.  Sums use float atomics, so the last bits vary
- * from run to run. */
+ * from run to run.  Frames that rtx_launch renders in probe order are
+ * replayed in the same kind of order. */
 rtx_status rtx_launch_adjoint(rtx_ctx* ctx, const rtx_lp_general* lp, const rtx_float3* dL_drgb, float* d_params,
                               void* stream);
 
