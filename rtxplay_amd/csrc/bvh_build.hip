@@ -31,9 +31,11 @@ namespace {
 constexpr int kBins = RTX_SAH_BINS;
 constexpr int kBuildBlock = 256;
 constexpr uint32_t kSmallTask = 256;   // primitives per task below which a level runs a wave per task
-constexpr uint32_t kWideTasks = 64;          // levels of at most this many tasks, averaging ...
-constexpr uint32_t kWideMinPrims = 65536;    // ... at least this many primitives, spread each task
-constexpr uint32_t kWideBlocks = 1024;       // over kWideBlocks / tasks chunks
+constexpr uint32_t kWideTasks = 512;      // levels of at most this many tasks, averaging ...
+constexpr uint32_t kWideMinPrims = 8192;  // ... at least this many primitives, spread each task
+constexpr uint32_t kWideBlocks = 1024;    // over kWideBlocks / tasks chunks, ...
+constexpr uint32_t kWideMinChunk = 2048;  // ... of this many primitives at least (on average)
+constexpr uint32_t kRootBlocks = 2048;                   // blocks of the root bounds reduction at most
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)
 #ifndef RTX_SAH_TRAV
@@ -129,14 +131,19 @@ __global__ void k_flatten(const InstBuild* __restrict__ insts, uint32_t ninst,
                          fmaxf(fmaxf(w[0].z, w[1].z), w[2].z), 0.f);
 }
 
-// root task: centroid bounds of all primitives (one block, strided)
-__global__ void k_root_task(const float4* __restrict__ plo, const float4* __restrict__ phi, uint32_t N,
-                            BuildTask* __restrict__ task, uint32_t* __restrict__ idx) {
-    __shared__ uint32_t s_lo[3], s_hi[3];
-    if (threadIdx.x < 3) { s_lo[threadIdx.x] = 0xFFFFFFFFu; s_hi[threadIdx.x] = 0u; }
+// root task: centroid bounds of all primitives as ordered ints (minima of x,
+// y, z, then maxima), reduced per block and merged across blocks by atomics;
+// min and max are exact, so the result does not depend on their order
+__global__ void k_root_init(uint32_t* __restrict__ rb) {
+    if (threadIdx.x < 6) rb[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+}
+__global__ __launch_bounds__(256) void k_root_bounds(const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                     uint32_t N, uint32_t* __restrict__ rb) {
+    __shared__ uint32_t s_b[6];
+    if (threadIdx.x < 6) s_b[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
     __syncthreads();
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-    for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         const float4 a = plo[p], b = phi[p];
         const float c[3] = {.5f * (a.x + b.x), .5f * (a.y + b.y), .5f * (a.z + b.z)};
         for (int k = 0; k < 3; k++) {
@@ -144,15 +151,18 @@ __global__ void k_root_task(const float4* __restrict__ plo, const float4* __rest
             hi[k] = max(hi[k], f2o(c[k]));
         }
     }
-    for (int k = 0; k < 3; k++) { atomicMin(&s_lo[k], lo[k]); atomicMax(&s_hi[k], hi[k]); }
+    for (int k = 0; k < 3; k++) { atomicMin(&s_b[k], lo[k]); atomicMax(&s_b[3 + k], hi[k]); }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        BuildTask t;
-        t.node = 0; t.begin = 0; t.end = N; t.depth = 0;
-        t.clo = make_float4(o2f(s_lo[0]), o2f(s_lo[1]), o2f(s_lo[2]), 0.f);
-        t.chi = make_float4(o2f(s_hi[0]), o2f(s_hi[1]), o2f(s_hi[2]), 0.f);
-        *task = t;
-    }
+    if (threadIdx.x < 3) atomicMin(&rb[threadIdx.x], s_b[threadIdx.x]);
+    else if (threadIdx.x < 6) atomicMax(&rb[threadIdx.x], s_b[threadIdx.x]);
+}
+__global__ void k_root_task(const uint32_t* __restrict__ rb, uint32_t N, BuildTask* __restrict__ task) {
+    if (threadIdx.x != 0) return;
+    BuildTask t;
+    t.node = 0; t.begin = 0; t.end = N; t.depth = 0;
+    t.clo = make_float4(o2f(rb[0]), o2f(rb[1]), o2f(rb[2]), 0.f);
+    t.chi = make_float4(o2f(rb[3]), o2f(rb[4]), o2f(rb[5]), 0.f);
+    *task = t;
 }
 
 __global__ void k_iota(uint32_t* __restrict__ idx, uint32_t N) {
@@ -514,8 +524,39 @@ __global__ __launch_bounds__(BLOCK) void k_bin_wide(const BuildTask* __restrict_
     for (uint32_t i = threadIdx.x; i < 3 * kBins * 12; i += BLOCK) P[3 * kBins + i] = bb[i];
 }
 
-// big tasks, step 2: one block per task merges the chunks' bins and decides
-// as k_level does; wsplit[t] = (kind, axis, bin, nleft) for the partition
+// big tasks, step 2a: the chunks' bins of each task merged into one set
+// (sums of counts, minima and maxima of bounds: exact, in any order).
+// Block (x, t) merges words 16x .. 16x+15 of task t: 16 groups of 16
+// threads each reduce every 16th chunk, then the first group the groups.
+constexpr uint32_t kMergeWords = 16;
+static_assert(kPartialWords % kMergeWords == 0, "whole blocks of merged words");
+__global__ __launch_bounds__(256) void k_merge_wide(uint32_t G, const uint32_t* __restrict__ partial,
+                                                    uint32_t* __restrict__ merged) {
+    __shared__ uint32_t s_v[256 / kMergeWords][kMergeWords + 1];
+    const uint32_t w = threadIdx.x % kMergeWords, grp = threadIdx.x / kMergeWords;
+    const uint32_t i = blockIdx.x * kMergeWords + w;
+    // word kind: 0 count (sum), 1 lower bound (min), 2 upper bound (max)
+    const int kind = i < 3 * kBins ? 0 : (bound_is_min((int)((i - 3 * kBins) % 12)) ? 1 : 2);
+    const uint32_t* P = partial + (size_t)blockIdx.y * G * kPartialWords + i;
+    uint32_t v = kind == 1 ? 0xFFFFFFFFu : 0u;
+    for (uint32_t k = grp; k < G; k += 256 / kMergeWords) {
+        const uint32_t x = P[(size_t)k * kPartialWords];
+        v = kind == 0 ? v + x : (kind == 1 ? min(v, x) : max(v, x));
+    }
+    s_v[grp][w] = v;
+    __syncthreads();
+    if (grp == 0) {
+        for (uint32_t g = 1; g < 256 / kMergeWords; g++) {
+            const uint32_t x = s_v[g][w];
+            v = kind == 0 ? v + x : (kind == 1 ? min(v, x) : max(v, x));
+        }
+        merged[(size_t)blockIdx.y * kPartialWords + i] = v;
+    }
+}
+
+// big tasks, step 2b: one block per task reads its G sets of bins (one, after
+// k_merge_wide), merges them and decides as k_level does; wsplit[t] = (kind,
+// axis, bin, nleft) for the partition
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_decide_wide(const BuildTask* __restrict__ tasks, uint32_t G,
                                                        const uint32_t* __restrict__ partial,
@@ -872,16 +913,24 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     RTX_TRY(bnodes.alloc(max_nodes));
     RTX_TRY(st.alloc(1));
     RTX_TRY(hipMemsetAsync(st.p, 0, sizeof(BuildStats), s));
-    DevBuf<uint32_t> partial, lcount;
+    DevBuf<uint32_t> partial, merged, lcount, rb;
     DevBuf<uint4> wsplit;
     if (N >= kWideMinPrims) {
-        RTX_TRY(partial.alloc((size_t)kWideBlocks * kPartialWords));
-        RTX_TRY(lcount.alloc(kWideBlocks));
-        RTX_TRY(wsplit.alloc(kWideTasks));
+        // chunks of a wide level: G * tasks <= N / kWideMinChunk; tasks <= N / kWideMinPrims
+        const uint32_t nchunks = std::min(kWideBlocks, N / kWideMinChunk);
+        const uint32_t nwide = std::min(kWideTasks, N / kWideMinPrims);
+        RTX_TRY(partial.alloc((size_t)nchunks * kPartialWords));
+        RTX_TRY(merged.alloc((size_t)nwide * kPartialWords));
+        RTX_TRY(lcount.alloc(nchunks));
+        RTX_TRY(wsplit.alloc(nwide));
     }
+    RTX_TRY(rb.alloc(6));
 
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(N, 256)), dim3(256), 0, s, idx.p, N);
-    hipLaunchKernelGGL(k_root_task, dim3(1), dim3(1024), 0, s, d_plo, d_phi, N, tasks_a.p, idx.p);
+    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(64), 0, s, rb.p);
+    hipLaunchKernelGGL(k_root_bounds, dim3(std::min<uint32_t>(blocks_for(N, 256), kRootBlocks)), dim3(256), 0, s,
+                       d_plo, d_phi, N, rb.p);
+    hipLaunchKernelGGL(k_root_task, dim3(1), dim3(64), 0, s, rb.p, N, tasks_a.p);
     RTX_TRY(hipGetLastError());
 
     // exclusive scan of so[0, n).kind into rank, the sum into total
@@ -903,15 +952,19 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     BuildTask* cur = tasks_a.p;
     BuildTask* nxt = tasks_b.p;
     while (ntasks > 0) {
-        // a few big tasks: each spread over kWideBlocks / ntasks chunks; a wave
-        // per task once tasks average fewer than kSmallTask primitives (the same
-        // tree either way: binning, SAH and partition do not depend on the split)
+        // a few big tasks: each spread over G chunks (kWideBlocks in all, of
+        // kWideMinChunk primitives at least on average), their bins merged by
+        // k_merge_wide; a wave per task once tasks average fewer than
+        // kSmallTask primitives (the same tree either way: binning, SAH and
+        // partition do not depend on the split)
         if (ntasks <= kWideTasks && (uint64_t)N >= (uint64_t)kWideMinPrims * ntasks) {
-            const uint32_t G = kWideBlocks / ntasks;
+            const uint32_t G = std::min(kWideBlocks / ntasks, N / (ntasks * kWideMinChunk));
             const dim3 gw(G, ntasks);
             hipLaunchKernelGGL(k_bin_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, G,
                                partial.p);
-            hipLaunchKernelGGL(k_decide_wide<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, G, partial.p,
+            hipLaunchKernelGGL(k_merge_wide, dim3(kPartialWords / kMergeWords, ntasks), dim3(256), 0, s, G, partial.p,
+                               merged.p);
+            hipLaunchKernelGGL(k_decide_wide<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, 1u, merged.p,
                                bnodes.p, so.p, wsplit.p, max_leaf);
             hipLaunchKernelGGL(k_count_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p, G,
                                wsplit.p, lcount.p);
