@@ -471,15 +471,12 @@ __device__ __forceinline__ void partition_range(const TaskGeom& g, int ax, int s
     }
 }
 
-// one block per task
+// one task on the whole block: bins, decision, stable partition, split record
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_level(
-        const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
-        const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
-        BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
-    __shared__ LevelLds L;
-    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
-    const BuildTask task = tasks[blockIdx.x];
+__device__ __forceinline__ void level_task(LevelLds& L, uint32_t* s_ws, const BuildTask& task, uint32_t slot,
+                                           const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                           uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+                                           BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
     const uint32_t n = task.end - task.begin;
     const TaskGeom g = task_geom(task);
     bins_clear<BLOCK>(L);
@@ -493,7 +490,185 @@ __global__ __launch_bounds__(BLOCK) void k_level(
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += BLOCK) idx[task.begin + i] = tmp[task.begin + i];
     }
-    split_out(L, task, out, blockIdx.x);
+    split_out(L, task, out, slot);
+}
+
+// one block per task
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_level(
+        const BuildTask* __restrict__ tasks, const float4* __restrict__ plo,
+        const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+        BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
+    __shared__ LevelLds L;
+    __shared__ uint32_t s_ws[BLOCK / 64 + 1];
+    level_task<BLOCK>(L, s_ws, tasks[blockIdx.x], blockIdx.x, plo, phi, idx, tmp, bnodes, out, max_leaf);
+}
+
+// ---------------------------------------------------------------------------
+// Levels of small tasks: a thread per task of at most kTinyTask primitives,
+// a wave per listed bigger task.  The thread makes the wave's decision: the
+// same bins, prefix and suffix bounds as minima and maxima over the same
+// primitives, the same cost expression and the same tie rule (the lowest
+// bin).  It evaluates only the splits after an occupied bin: a split after
+// an empty bin partitions like the split after the occupied bin below it and
+// has the same cost at a higher bin, so it never wins.  (A minimum over
+// zeros of both signs may keep the other sign than the wave's; the boxes and
+// SAH costs are equal as numbers either way.)
+constexpr uint32_t kTinyTask = 16;
+constexpr uint32_t kTinyLevel = 8192;  // tasks of a level below which a wave per task runs them all (fewer launches)
+constexpr uint32_t kListGrid = 4096;   // waves of the listed-task kernel at most
+
+__device__ __forceinline__ float centroid(const float4& a, const float4& h, int ax) {
+    return .5f * (comp4(a, ax) + comp4(h, ax));
+}
+
+__device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __restrict__ plo,
+                          const float4* __restrict__ phi, uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+                          BNode* __restrict__ bnodes, SplitOut* __restrict__ out, uint32_t max_leaf) {
+    const uint32_t n = task.end - task.begin;
+    const TaskGeom g = task_geom(task);
+    const uint32_t* ix = idx + task.begin;
+    float nlx = FLT_MAX, nly = FLT_MAX, nlz = FLT_MAX, nhx = -FLT_MAX, nhy = -FLT_MAX, nhz = -FLT_MAX;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t p = ix[i];
+        const float4 a = plo[p], h = phi[p];
+        nlx = fminf(nlx, a.x); nly = fminf(nly, a.y); nlz = fminf(nlz, a.z);
+        nhx = fmaxf(nhx, h.x); nhy = fmaxf(nhy, h.y); nhz = fmaxf(nhz, h.z);
+    }
+    float best_cost[3];
+    int best_bin[3];
+    for (int ax = 0; ax < 3; ax++) {
+        uint64_t key = ~0ull;
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t pj = ix[j];
+            const int b = bin_of(centroid(plo[pj], phi[pj], ax), g.cl[ax], g.scale[ax]);
+            if (b >= kBins - 1) continue;   // no bin right of it
+            float llx = FLT_MAX, lly = FLT_MAX, llz = FLT_MAX, lhx = -FLT_MAX, lhy = -FLT_MAX, lhz = -FLT_MAX;
+            float rlx = FLT_MAX, rly = FLT_MAX, rlz = FLT_MAX, rhx = -FLT_MAX, rhy = -FLT_MAX, rhz = -FLT_MAX;
+            uint32_t pc = 0, rc = 0;
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t p = ix[i];
+                const float4 a = plo[p], h = phi[p];
+                if (bin_of(centroid(a, h, ax), g.cl[ax], g.scale[ax]) <= b) {
+                    llx = fminf(llx, a.x); lly = fminf(lly, a.y); llz = fminf(llz, a.z);
+                    lhx = fmaxf(lhx, h.x); lhy = fmaxf(lhy, h.y); lhz = fmaxf(lhz, h.z);
+                    pc++;
+                } else {
+                    rlx = fminf(rlx, a.x); rly = fminf(rly, a.y); rlz = fminf(rlz, a.z);
+                    rhx = fmaxf(rhx, h.x); rhy = fmaxf(rhy, h.y); rhz = fmaxf(rhz, h.z);
+                    rc++;
+                }
+            }
+            const float cost = area3(llx, lly, llz, lhx, lhy, lhz) * (float)pc +
+                               area3(rlx, rly, rlz, rhx, rhy, rhz) * (float)rc;
+            if (pc != 0 && rc != 0 && cost < FLT_MAX) {
+                const uint64_t k = ((uint64_t)__float_as_uint(cost) << 32) | (uint32_t)b;
+                key = k < key ? k : key;
+            }
+        }
+        best_cost[ax] = key == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(key >> 32));
+        best_bin[ax] = key == ~0ull ? -1 : (int)(uint32_t)key;
+    }
+    BNode nd;
+    nd.lo = make_float4(nlx, nly, nlz, 0.f);
+    nd.hi = make_float4(nhx, nhy, nhz, 0.f);
+    bnodes[task.node] = nd;
+    int axis = -1;
+    float best = FLT_MAX;
+    for (int ax = 0; ax < 3; ax++)
+        if (best_bin[ax] >= 0 && best_cost[ax] < best) { best = best_cost[ax]; axis = ax; }
+    const float A = area3(nlx, nly, nlz, nhx, nhy, nhz);
+    const float leaf_cost = kCostIsect * (float)n * A;
+    const float split_cost = kCostTrav * A + kCostIsect * best;
+    int kind;   // 0 leaf, 1 SAH split, 2 median split (as in sah_decide)
+    if (n <= 1) kind = 0;
+    else if (n <= max_leaf && (axis < 0 || leaf_cost <= split_cost)) kind = 0;
+    else if (axis < 0 || task.depth >= (uint32_t)(kMaxTreeDepth - 28)) kind = 2;
+    else kind = 1;
+    SplitOut so;
+    so.kind = kind == 0 ? 0u : 1u;
+    so.pad0 = so.pad1 = 0;
+    if (kind == 1) {
+        // stable partition into tmp and back, with the centroid bounds of both sides
+        const int sb = best_bin[axis];
+        float llx = FLT_MAX, lly = FLT_MAX, llz = FLT_MAX, lhx = -FLT_MAX, lhy = -FLT_MAX, lhz = -FLT_MAX;
+        float rlx = FLT_MAX, rly = FLT_MAX, rlz = FLT_MAX, rhx = -FLT_MAX, rhy = -FLT_MAX, rhz = -FLT_MAX;
+        uint32_t nl = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t p = ix[i];
+            const float4 a = plo[p], h = phi[p];
+            const float cx = centroid(a, h, 0), cy = centroid(a, h, 1), cz = centroid(a, h, 2);
+            const float ca = axis == 0 ? cx : (axis == 1 ? cy : cz);
+            if (bin_of(ca, g.cl[axis], g.scale[axis]) <= sb) {
+                llx = fminf(llx, cx); lly = fminf(lly, cy); llz = fminf(llz, cz);
+                lhx = fmaxf(lhx, cx); lhy = fmaxf(lhy, cy); lhz = fmaxf(lhz, cz);
+                tmp[task.begin + nl++] = p;
+            } else {
+                rlx = fminf(rlx, cx); rly = fminf(rly, cy); rlz = fminf(rlz, cz);
+                rhx = fmaxf(rhx, cx); rhy = fmaxf(rhy, cy); rhz = fmaxf(rhz, cz);
+            }
+        }
+        uint32_t r = nl;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t p = ix[i];
+            if (bin_of(centroid(plo[p], phi[p], axis), g.cl[axis], g.scale[axis]) > sb) tmp[task.begin + r++] = p;
+        }
+        for (uint32_t i = 0; i < n; i++) idx[task.begin + i] = tmp[task.begin + i];
+        so.nleft = nl;
+        so.clo_l = make_float4(llx, lly, llz, 0.f);
+        so.chi_l = make_float4(lhx, lhy, lhz, 0.f);
+        so.clo_r = make_float4(rlx, rly, rlz, 0.f);
+        so.chi_r = make_float4(rhx, rhy, rhz, 0.f);
+    } else {
+        so.nleft = n / 2;
+        so.clo_l = so.clo_r = task.clo;
+        so.chi_l = so.chi_r = task.chi;
+    }
+    out[slot] = so;
+}
+
+// a thread per task; tasks above kTinyTask primitives are appended to `big`
+// (one atomic per wave: their order in the list does not matter, each task
+// writes only its own slot and primitive range)
+__global__ __launch_bounds__(256) void k_level_tiny(const BuildTask* __restrict__ tasks, uint32_t ntasks,
+                                                    const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                    uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+                                                    BNode* __restrict__ bnodes, SplitOut* __restrict__ out,
+                                                    uint32_t max_leaf, uint32_t* __restrict__ big,
+                                                    uint32_t* __restrict__ nbig) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool is_big = false;
+    if (t < ntasks) {
+        const BuildTask task = tasks[t];
+        if (task.end - task.begin > kTinyTask) is_big = true;
+        else tiny_task(task, t, plo, phi, idx, tmp, bnodes, out, max_leaf);
+    }
+    const uint64_t m = __ballot(is_big);
+    if (m) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int first = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == (uint32_t)first) base = atomicAdd(nbig, (uint32_t)__popcll(m));
+        base = __shfl(base, first, 64);
+        if (is_big) big[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+    }
+}
+
+// a wave per listed task, grid-strided over the list
+__global__ __launch_bounds__(64) void k_level_list(const BuildTask* __restrict__ tasks,
+                                                   const uint32_t* __restrict__ big, const uint32_t* __restrict__ nbig,
+                                                   const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                   uint32_t* __restrict__ idx, uint32_t* __restrict__ tmp,
+                                                   BNode* __restrict__ bnodes, SplitOut* __restrict__ out,
+                                                   uint32_t max_leaf) {
+    __shared__ LevelLds L;
+    __shared__ uint32_t s_ws[2];
+    const uint32_t nb = *nbig;
+    for (uint32_t k = blockIdx.x; k < nb; k += gridDim.x) {
+        const uint32_t t = big[k];
+        level_task<64>(L, s_ws, tasks[t], t, plo, phi, idx, tmp, bnodes, out, max_leaf);
+        __syncthreads();
+    }
 }
 
 // big tasks, step 1: block (chunk c, task t) bins chunk c of the task into
@@ -925,6 +1100,10 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         RTX_TRY(wsplit.alloc(nwide));
     }
     RTX_TRY(rb.alloc(6));
+    // tasks of a small-task level above kTinyTask primitives (at most N / (kTinyTask + 1))
+    DevBuf<uint32_t> big, nbig;
+    RTX_TRY(big.alloc(N / (kTinyTask + 1) + 1));
+    RTX_TRY(nbig.alloc(1));
 
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(N, 256)), dim3(256), 0, s, idx.p, N);
     hipLaunchKernelGGL(k_root_init, dim3(1), dim3(64), 0, s, rb.p);
@@ -955,8 +1134,9 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         // a few big tasks: each spread over G chunks (kWideBlocks in all, of
         // kWideMinChunk primitives at least on average), their bins merged by
         // k_merge_wide; a wave per task once tasks average fewer than
-        // kSmallTask primitives (the same tree either way: binning, SAH and
-        // partition do not depend on the split)
+        // kSmallTask primitives, and on levels of kTinyLevel tasks or more a
+        // thread per task of at most kTinyTask primitives (the same tree every
+        // way: binning, SAH and partition do not depend on the split)
         if (ntasks <= kWideTasks && (uint64_t)N >= (uint64_t)kWideMinPrims * ntasks) {
             const uint32_t G = std::min(kWideBlocks / ntasks, N / (ntasks * kWideMinChunk));
             const dim3 gw(G, ntasks);
@@ -971,10 +1151,16 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
             hipLaunchKernelGGL(k_scatter_wide<kBuildBlock>, gw, dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p,
                                tmp.p, G, wsplit.p, lcount.p);
             hipLaunchKernelGGL(k_copy_wide, gw, dim3(256), 0, s, cur, G, wsplit.p, tmp.p, idx.p);
-        } else if ((uint64_t)N < (uint64_t)kSmallTask * ntasks)
+        } else if ((uint64_t)N < (uint64_t)kSmallTask * ntasks && ntasks < kTinyLevel) {
             hipLaunchKernelGGL(k_level<64>, dim3(ntasks), dim3(64), 0, s, cur, d_plo, d_phi, idx.p, tmp.p, bnodes.p,
                                so.p, max_leaf);
-        else
+        } else if ((uint64_t)N < (uint64_t)kSmallTask * ntasks) {
+            RTX_TRY(hipMemsetAsync(nbig.p, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_level_tiny, dim3(blocks_for(ntasks, 256)), dim3(256), 0, s, cur, ntasks, d_plo, d_phi,
+                               idx.p, tmp.p, bnodes.p, so.p, max_leaf, big.p, nbig.p);
+            hipLaunchKernelGGL(k_level_list, dim3(std::min(ntasks, kListGrid)), dim3(64), 0, s, cur, big.p, nbig.p,
+                               d_plo, d_phi, idx.p, tmp.p, bnodes.p, so.p, max_leaf);
+        } else
             hipLaunchKernelGGL(k_level<kBuildBlock>, dim3(ntasks), dim3(kBuildBlock), 0, s, cur, d_plo, d_phi, idx.p,
                                tmp.p, bnodes.p, so.p, max_leaf);
         scan_kind(ntasks);
