@@ -61,6 +61,18 @@ def test_c3_flatten_bit_exact(c3):
     assert np.array_equal(gi, inst)
 
 
+def test_c3_tree_is_unchanged(c3):
+    """The builder's config-3 tree, pinned like the level-4 one: its build runs
+    the chunked (wide) levels, a block per task, a wave per task and, on the
+    deepest levels, a thread per task of at most 16 primitives; every path
+    makes the same decisions, so the tree is the one the builder gave before
+    the thread-per-task path and the merged wide bins existed (recorded from
+    it)."""
+    st = c3[0].stats()
+    assert (st["num_nodes"], st["num_leaves"], st["max_depth"]) == (4625294, 4625295, 31)
+    assert abs(st["sah_cost"] - 37.87184524536133) < 1e-9
+
+
 def test_c3_hit_ids(orc, c3):
     s, tris, inst, bvh = c3
     cam = orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, W / H, 0.1, 10.0)

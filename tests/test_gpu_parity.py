@@ -91,7 +91,8 @@ def test_build_level4_stats(level4):
 
 def test_build_tree_is_unchanged(level4):
     """The builder's tree for the level-4 scene, pinned: the block-per-task,
-    wave-per-task and chunked (wide) level paths must all give this tree
+    wave-per-task, thread-per-task (tasks of at most 16 primitives on levels
+    of 8192 tasks or more) and chunked (wide) level paths must all give this tree
     (binning is sums, minima and maxima, and the partition is stable), so a
     change of any of them that alters the tree fails here.  The values are
     the builder's own output, recorded when the paths were introduced."""
