@@ -876,27 +876,44 @@ __global__ void k_emit(const BuildTask* __restrict__ tasks, const SplitOut* __re
                        BNode* __restrict__ bnodes, BuildTask* __restrict__ next,
                        BuildStats* __restrict__ st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntasks) return;
-    const BuildTask t = tasks[i];
-    const SplitOut s = so[i];
-    const uint32_t n = t.end - t.begin;
-    if (s.kind == 0) {
-        bnodes[t.node].lo.w = __uint_as_float(t.begin);
-        bnodes[t.node].hi.w = __uint_as_float(n);
-        atomicAdd(&st->leaves, 1u);
-        atomicMax(&st->max_depth, t.depth + 1);
-        atomicMax(&st->max_leaf, n);
-    } else {
-        const uint32_t L = node_base + 2 * rank[i];
-        bnodes[t.node].lo.w = __uint_as_float(L);
-        bnodes[t.node].hi.w = __uint_as_float(0u);
-        BuildTask a, b;
-        a.node = L; a.begin = t.begin; a.end = t.begin + s.nleft; a.depth = t.depth + 1;
-        a.clo = s.clo_l; a.chi = s.chi_l;
-        b.node = L + 1; b.begin = t.begin + s.nleft; b.end = t.end; b.depth = t.depth + 1;
-        b.clo = s.clo_r; b.chi = s.chi_r;
-        next[2 * rank[i]] = a;
-        next[2 * rank[i] + 1] = b;
+    uint32_t leaf_depth = 0, leaf_n = 0;   // of a leaf, for the statistics
+    bool leaf = false;
+    if (i < ntasks) {
+        const BuildTask t = tasks[i];
+        const SplitOut s = so[i];
+        const uint32_t n = t.end - t.begin;
+        if (s.kind == 0) {
+            bnodes[t.node].lo.w = __uint_as_float(t.begin);
+            bnodes[t.node].hi.w = __uint_as_float(n);
+            leaf = true;
+            leaf_depth = t.depth + 1;
+            leaf_n = n;
+        } else {
+            const uint32_t L = node_base + 2 * rank[i];
+            bnodes[t.node].lo.w = __uint_as_float(L);
+            bnodes[t.node].hi.w = __uint_as_float(0u);
+            BuildTask a, b;
+            a.node = L; a.begin = t.begin; a.end = t.begin + s.nleft; a.depth = t.depth + 1;
+            a.clo = s.clo_l; a.chi = s.chi_l;
+            b.node = L + 1; b.begin = t.begin + s.nleft; b.end = t.end; b.depth = t.depth + 1;
+            b.clo = s.clo_r; b.chi = s.chi_r;
+            next[2 * rank[i]] = a;
+            next[2 * rank[i] + 1] = b;
+        }
+    }
+    // leaf count, deepest leaf and largest leaf with one atomic each per wave
+    // (millions of leaves on one address otherwise serialise in L2)
+    const uint64_t m = __ballot(leaf);
+    if (m == 0) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        leaf_depth = max(leaf_depth, (uint32_t)__shfl_xor(leaf_depth, off, 64));
+        leaf_n = max(leaf_n, (uint32_t)__shfl_xor(leaf_n, off, 64));
+    }
+    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)m) - 1)) {
+        atomicAdd(&st->leaves, (uint32_t)__popcll(m));
+        atomicMax(&st->max_depth, leaf_depth);
+        atomicMax(&st->max_leaf, leaf_n);
     }
 }
 
