@@ -514,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(
 // has the same cost at a higher bin, so it never wins.  (A minimum over
 // zeros of both signs may keep the other sign than the wave's; the boxes and
 // SAH costs are equal as numbers either way.)
-constexpr uint32_t kTinyTask = 16;
+constexpr uint32_t kTinyTask = 16;     // a power of two (the bitonic sort), at most 16 (4 position bits)
 constexpr uint32_t kTinyLevel = 8192;  // tasks of a level below which a wave per task runs them all (fewer launches)
 constexpr uint32_t kListGrid = 4096;   // waves of the listed-task kernel at most
 
@@ -538,36 +538,74 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
     float best_cost[3];
     int best_bin[3];
     for (int ax = 0; ax < 3; ax++) {
-        uint64_t key = ~0ull;
-        for (uint32_t j = 0; j < n; j++) {
-            const uint32_t pj = ix[j];
-            const int b = bin_of(centroid(plo[pj], phi[pj], ax), g.cl[ax], g.scale[ax]);
-            if (b >= kBins - 1) continue;   // no bin right of it
-            float llx = FLT_MAX, lly = FLT_MAX, llz = FLT_MAX, lhx = -FLT_MAX, lhy = -FLT_MAX, lhz = -FLT_MAX;
-            float rlx = FLT_MAX, rly = FLT_MAX, rlz = FLT_MAX, rhx = -FLT_MAX, rhy = -FLT_MAX, rhz = -FLT_MAX;
-            uint32_t pc = 0, rc = 0;
-            for (uint32_t i = 0; i < n; i++) {
+        // the primitives in order of their bin (bin << 4 | position, a bitonic
+        // sort over kTinyTask register slots; absent slots sort last)
+        uint32_t key[kTinyTask];
+#pragma unroll
+        for (uint32_t i = 0; i < kTinyTask; i++) {
+            key[i] = 0xFFFFFFFFu;
+            if (i < n) {
                 const uint32_t p = ix[i];
-                const float4 a = plo[p], h = phi[p];
-                if (bin_of(centroid(a, h, ax), g.cl[ax], g.scale[ax]) <= b) {
-                    llx = fminf(llx, a.x); lly = fminf(lly, a.y); llz = fminf(llz, a.z);
-                    lhx = fmaxf(lhx, h.x); lhy = fmaxf(lhy, h.y); lhz = fmaxf(lhz, h.z);
-                    pc++;
-                } else {
-                    rlx = fminf(rlx, a.x); rly = fminf(rly, a.y); rlz = fminf(rlz, a.z);
-                    rhx = fmaxf(rhx, h.x); rhy = fmaxf(rhy, h.y); rhz = fmaxf(rhz, h.z);
-                    rc++;
-                }
-            }
-            const float cost = area3(llx, lly, llz, lhx, lhy, lhz) * (float)pc +
-                               area3(rlx, rly, rlz, rhx, rhy, rhz) * (float)rc;
-            if (pc != 0 && rc != 0 && cost < FLT_MAX) {
-                const uint64_t k = ((uint64_t)__float_as_uint(cost) << 32) | (uint32_t)b;
-                key = k < key ? k : key;
+                key[i] = ((uint32_t)bin_of(centroid(plo[p], phi[p], ax), g.cl[ax], g.scale[ax]) << 4) | i;
             }
         }
-        best_cost[ax] = key == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(key >> 32));
-        best_bin[ax] = key == ~0ull ? -1 : (int)(uint32_t)key;
+#pragma unroll
+        for (uint32_t k = 2; k <= kTinyTask; k <<= 1) {
+#pragma unroll
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (uint32_t i = 0; i < kTinyTask; i++) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint32_t a = key[i], b = key[l];
+                        const bool lo_first = (i & k) == 0;
+                        key[i] = lo_first ? min(a, b) : max(a, b);
+                        key[l] = lo_first ? max(a, b) : min(a, b);
+                    }
+                }
+            }
+        }
+        // prefix terms area(bins <= bin of position k) * count and suffix terms
+        // area(bins of positions >= k) * count, as the wave's scans form them
+        float tl[kTinyTask], tr[kTinyTask];
+        float lx0 = FLT_MAX, ly0 = FLT_MAX, lz0 = FLT_MAX, lx1 = -FLT_MAX, ly1 = -FLT_MAX, lz1 = -FLT_MAX;
+#pragma unroll
+        for (uint32_t k = 0; k < kTinyTask; k++) {
+            tl[k] = 0.f;
+            if (k < n) {
+                const uint32_t p = ix[key[k] & 15u];
+                const float4 a = plo[p], h = phi[p];
+                lx0 = fminf(lx0, a.x); ly0 = fminf(ly0, a.y); lz0 = fminf(lz0, a.z);
+                lx1 = fmaxf(lx1, h.x); ly1 = fmaxf(ly1, h.y); lz1 = fmaxf(lz1, h.z);
+                tl[k] = area3(lx0, ly0, lz0, lx1, ly1, lz1) * (float)(k + 1);
+            }
+        }
+        float rx0 = FLT_MAX, ry0 = FLT_MAX, rz0 = FLT_MAX, rx1 = -FLT_MAX, ry1 = -FLT_MAX, rz1 = -FLT_MAX;
+#pragma unroll
+        for (int k = (int)kTinyTask - 1; k >= 0; k--) {
+            tr[k] = 0.f;
+            if ((uint32_t)k < n) {
+                const uint32_t p = ix[key[k] & 15u];
+                const float4 a = plo[p], h = phi[p];
+                rx0 = fminf(rx0, a.x); ry0 = fminf(ry0, a.y); rz0 = fminf(rz0, a.z);
+                rx1 = fmaxf(rx1, h.x); ry1 = fmaxf(ry1, h.y); rz1 = fmaxf(rz1, h.z);
+                tr[k] = area3(rx0, ry0, rz0, rx1, ry1, rz1) * (float)(n - (uint32_t)k);
+            }
+        }
+        // a split after bin b of position k: position k + 1 holds a higher bin
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (uint32_t k = 0; k + 1 < kTinyTask; k++) {
+            if (k + 1 < n && (key[k] >> 4) != (key[k + 1] >> 4)) {
+                const float cost = tl[k] + tr[k + 1];
+                if (cost < FLT_MAX) {
+                    const uint64_t c = ((uint64_t)__float_as_uint(cost) << 32) | (key[k] >> 4);
+                    best = c < best ? c : best;
+                }
+            }
+        }
+        best_cost[ax] = best == ~0ull ? FLT_MAX : __uint_as_float((uint32_t)(best >> 32));
+        best_bin[ax] = best == ~0ull ? -1 : (int)(uint32_t)best;
     }
     BNode nd;
     nd.lo = make_float4(nlx, nly, nlz, 0.f);
