@@ -514,7 +514,11 @@ __global__ __launch_bounds__(BLOCK) void k_level(
 // has the same cost at a higher bin, so it never wins.  (A minimum over
 // zeros of both signs may keep the other sign than the wave's; the boxes and
 // SAH costs are equal as numbers either way.)
-constexpr uint32_t kTinyTask = 16;     // a power of two (the bitonic sort), at most 16 (4 position bits)
+#ifndef RTX_TINY_BITS
+#define RTX_TINY_BITS 4
+#endif
+constexpr uint32_t kTinyBits = RTX_TINY_BITS;     // position bits of a sort key
+constexpr uint32_t kTinyTask = 1u << kTinyBits;   // a power of two (the bitonic sort)
 constexpr uint32_t kTinyLevel = 8192;  // tasks of a level below which a wave per task runs them all (fewer launches)
 constexpr uint32_t kListGrid = 4096;   // waves of the listed-task kernel at most
 
@@ -538,7 +542,7 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
     float best_cost[3];
     int best_bin[3];
     for (int ax = 0; ax < 3; ax++) {
-        // the primitives in order of their bin (bin << 4 | position, a bitonic
+        // the primitives in order of their bin (bin << kTinyBits | position, a bitonic
         // sort over kTinyTask register slots; absent slots sort last)
         uint32_t key[kTinyTask];
 #pragma unroll
@@ -546,7 +550,7 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
             key[i] = 0xFFFFFFFFu;
             if (i < n) {
                 const uint32_t p = ix[i];
-                key[i] = ((uint32_t)bin_of(centroid(plo[p], phi[p], ax), g.cl[ax], g.scale[ax]) << 4) | i;
+                key[i] = ((uint32_t)bin_of(centroid(plo[p], phi[p], ax), g.cl[ax], g.scale[ax]) << kTinyBits) | i;
             }
         }
 #pragma unroll
@@ -573,7 +577,7 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
         for (uint32_t k = 0; k < kTinyTask; k++) {
             tl[k] = 0.f;
             if (k < n) {
-                const uint32_t p = ix[key[k] & 15u];
+                const uint32_t p = ix[key[k] & (kTinyTask - 1u)];
                 const float4 a = plo[p], h = phi[p];
                 lx0 = fminf(lx0, a.x); ly0 = fminf(ly0, a.y); lz0 = fminf(lz0, a.z);
                 lx1 = fmaxf(lx1, h.x); ly1 = fmaxf(ly1, h.y); lz1 = fmaxf(lz1, h.z);
@@ -585,7 +589,7 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
         for (int k = (int)kTinyTask - 1; k >= 0; k--) {
             tr[k] = 0.f;
             if ((uint32_t)k < n) {
-                const uint32_t p = ix[key[k] & 15u];
+                const uint32_t p = ix[key[k] & (kTinyTask - 1u)];
                 const float4 a = plo[p], h = phi[p];
                 rx0 = fminf(rx0, a.x); ry0 = fminf(ry0, a.y); rz0 = fminf(rz0, a.z);
                 rx1 = fmaxf(rx1, h.x); ry1 = fmaxf(ry1, h.y); rz1 = fmaxf(rz1, h.z);
@@ -596,10 +600,10 @@ __device__ void tiny_task(const BuildTask& task, uint32_t slot, const float4* __
         uint64_t best = ~0ull;
 #pragma unroll
         for (uint32_t k = 0; k + 1 < kTinyTask; k++) {
-            if (k + 1 < n && (key[k] >> 4) != (key[k + 1] >> 4)) {
+            if (k + 1 < n && (key[k] >> kTinyBits) != (key[k + 1] >> kTinyBits)) {
                 const float cost = tl[k] + tr[k + 1];
                 if (cost < FLT_MAX) {
-                    const uint64_t c = ((uint64_t)__float_as_uint(cost) << 32) | (key[k] >> 4);
+                    const uint64_t c = ((uint64_t)__float_as_uint(cost) << 32) | (key[k] >> kTinyBits);
                     best = c < best ? c : best;
                 }
             }
