@@ -885,27 +885,36 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_kind(const SplitOut* __rest
 // the same scan over many blocks, for long arrays: tile sums, a scan of
 // them (k_scan_u32), then each tile rescanned from its offset
 constexpr uint32_t kScanTile = 4 * kScanBlock;
-__global__ __launch_bounds__(kScanBlock) void k_kind_tiles(const SplitOut* __restrict__ so, uint32_t n,
-                                                          uint32_t* __restrict__ tsum) {
+// the values to scan: the split flags of a level, or plain counts
+struct KindOf {
+    const SplitOut* so;
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return so[i].kind; }
+};
+struct U32Of {
+    const uint32_t* p;
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return p[i]; }
+};
+template <class Get>
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(Get get, uint32_t n, uint32_t* __restrict__ tsum) {
     __shared__ uint32_t s_n;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     uint32_t c = 0;
     const uint32_t b0 = blockIdx.x * kScanTile;
-    for (uint32_t i = b0 + threadIdx.x; i < n && i < b0 + kScanTile; i += kScanBlock) c += so[i].kind;
+    for (uint32_t i = b0 + threadIdx.x; i < n && i < b0 + kScanTile; i += kScanBlock) c += get(i);
     atomicAdd(&s_n, c);
     __syncthreads();
     if (threadIdx.x == 0) tsum[blockIdx.x] = s_n;
 }
-__global__ __launch_bounds__(kScanBlock) void k_kind_apply(const SplitOut* __restrict__ so, uint32_t n,
-                                                          const uint32_t* __restrict__ toff,
+template <class Get>
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(Get get, uint32_t n, const uint32_t* __restrict__ toff,
                                                           uint32_t* __restrict__ out) {
     __shared__ uint32_t s_ws[kScanBlock / 64 + 1];
     uint32_t carry = toff[blockIdx.x];
     const uint32_t b0 = blockIdx.x * kScanTile;
     for (uint32_t base = b0; base < b0 + kScanTile && base < n; base += kScanBlock) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < n ? so[i].kind : 0u;
+        const uint32_t v = i < n ? get(i) : 0u;
         uint32_t tot;
         const uint32_t ex = block_scan<kScanBlock>(v, s_ws, tot);
         if (i < n) out[i] = carry + ex;
@@ -960,18 +969,28 @@ __global__ void k_emit(const BuildTask* __restrict__ tasks, const SplitOut* __re
 }
 
 // SAH cost of the finished tree (relative to the root area)
-__global__ void k_sah(const BNode* __restrict__ bnodes, uint32_t nnodes, BuildStats* __restrict__ st) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// SAH cost of the binary tree (statistics): grid-strided partial sums, a wave
+// reduction, then one atomic per block (the order of the double sums varies
+// from run to run in the last bits)
+constexpr uint32_t kSahBlocks = 1024;
+__global__ __launch_bounds__(256) void k_sah(const BNode* __restrict__ bnodes, uint32_t nnodes,
+                                             BuildStats* __restrict__ st) {
+    __shared__ double s_v[256 / 64];
     double v = 0.0;
-    if (i < nnodes) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nnodes; i += gridDim.x * blockDim.x) {
         const BNode nd = bnodes[i];
         const float a = area3(nd.lo.x, nd.lo.y, nd.lo.z, nd.hi.x, nd.hi.y, nd.hi.z);
         const uint32_t cnt = __float_as_uint(nd.hi.w);
-        v = cnt ? (double)kCostIsect * cnt * a : (double)kCostTrav * a;
+        v += cnt ? (double)kCostIsect * cnt * a : (double)kCostTrav * a;
     }
-    // wave reduction, then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&st->sah, v);
+    if ((threadIdx.x & 63) == 0) s_v[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / 64; w++) t += s_v[w];
+        if (t != 0.0) atomicAdd(&st->sah, t);
+    }
 }
 
 __global__ void k_inner_flag(const BNode* __restrict__ bnodes, uint32_t nnodes, SplitOut* __restrict__ flag) {
@@ -1181,9 +1200,20 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
             return;
         }
         const uint32_t nt = (n + kScanTile - 1) / kScanTile;
-        hipLaunchKernelGGL(k_kind_tiles, dim3(nt), dim3(kScanBlock), 0, s, so.p, n, tsum.p);
+        hipLaunchKernelGGL(k_scan_tiles<KindOf>, dim3(nt), dim3(kScanBlock), 0, s, KindOf{so.p}, n, tsum.p);
         hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, tsum.p, nt, toff.p, total.p);
-        hipLaunchKernelGGL(k_kind_apply, dim3(nt), dim3(kScanBlock), 0, s, so.p, n, toff.p, rank.p);
+        hipLaunchKernelGGL(k_scan_apply<KindOf>, dim3(nt), dim3(kScanBlock), 0, s, KindOf{so.p}, n, toff.p, rank.p);
+    };
+    // exclusive scan of in[0, n) into out, the sum into total (the same tiles)
+    auto scan_u32 = [&](const uint32_t* in, uint32_t n, uint32_t* out) {
+        if (n <= 16 * kScanTile) {
+            hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, in, n, out, total.p);
+            return;
+        }
+        const uint32_t nt = (n + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(k_scan_tiles<U32Of>, dim3(nt), dim3(kScanBlock), 0, s, U32Of{in}, n, tsum.p);
+        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, tsum.p, nt, toff.p, total.p);
+        hipLaunchKernelGGL(k_scan_apply<U32Of>, dim3(nt), dim3(kScanBlock), 0, s, U32Of{in}, n, toff.p, out);
     };
 
     uint32_t ntasks = 1, nnodes = 1;
@@ -1234,7 +1264,8 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         std::swap(cur, nxt);
     }
 
-    hipLaunchKernelGGL(k_sah, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, st.p);
+    hipLaunchKernelGGL(k_sah, dim3(std::min(blocks_for(nnodes, 256), kSahBlocks)), dim3(256), 0, s, bnodes.p, nnodes,
+                       st.p);
     // compact inner nodes into the traversal layout (BFS order of the build)
     hipLaunchKernelGGL(k_inner_flag, dim3(blocks_for(nnodes, 256)), dim3(256), 0, s, bnodes.p, nnodes, so.p);
     scan_kind(nnodes);
@@ -1276,7 +1307,7 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
             out.level_begin.push_back(nwide);
             hipLaunchKernelGGL(k_collapse4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cur, n, cid.p,
                                cnt.p);
-            hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kScanBlock), 0, s, cnt.p, n, off.p, total.p);
+            scan_u32(cnt.p, n, off.p);
             hipLaunchKernelGGL(k_write4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cid.p, off.p, n,
                                nwide, nwide + n, leaf_ref(N, 1), wide.p, nxt);
             RTX_TRY(hipGetLastError());
