@@ -227,6 +227,25 @@ rtx_status rtx_tuning_set(rtx_ctx* ctx, const rtx_tuning* t);
 /* Miss record (SBT miss data), optx/rtwo.cxx:110-111 and :474-489.  Default {.5,.7,1}. */
 rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
 
+/* Which of the reference's two program sets a context's launches follow
+ * (optx/rtwo.cxx:44-50 picks one at compile time; optx/Makefile:57-59).
+ * RTX_SEMANTICS_RECURSIVE (the default here) is the -DRECURSIVE build
+ * (optx/camera_r.cu, optx/optics_r.cu): a hit by the lp.depth-th ray of a
+ * sample is black and draws nothing (optx/optics_r.cu:31-42); the guide
+ * values are the deepest diffuse or metal hit's normal and the deepest
+ * non-zero albedo (set after the recursion returns, optx/optics_r.cu:143-146).
+ * RTX_SEMANTICS_ITERATIVE is the default build (optx/camera_i.cu,
+ * optx/optics_i.cu): that hit is shaded like any other (its random draws
+ * included) and the sample keeps the throughput as its colour
+ * (optx/camera_i.cu:75-95); the guide values are the first diffuse or metal
+ * hit's normal and the first non-zero albedo (optx/optics_i.cu:98-101).
+ * Misses, ray counts and every other step are the same.  Applies to
+ * rtx_launch, rtx_launch_counted, rtx_launch_adjoint and batches launched
+ * after the call. */
+enum { RTX_SEMANTICS_RECURSIVE = 0, RTX_SEMANTICS_ITERATIVE = 1 };
+rtx_status rtx_set_semantics(rtx_ctx* ctx, int mode);
+rtx_status rtx_get_semantics(rtx_ctx* ctx, int* mode);
+
 /* ---- launch -------------------------------------------------------------- */
 /* Launcher::ignite, optx/launcher.cxx:49-77 (optixLaunch of the raygen program
  * optx/camera_r.cu:23-121 with the closest-hit programs optx/optics_r.cu).
@@ -237,6 +256,7 @@ rtx_status rtx_set_miss_color(rtx_ctx* ctx, rtx_float3 ambient);
  * probe's rays (the pixels do not depend on the order); the first such launch
  * of a context allocates the order's arrays (8 bytes per tile). */
 rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
+
 
 /* Batched render (no reference counterpart; SURVEY config 4): n frames, each
  * with its own launch parameters (scene handle, camera, size, spp, depth,

@@ -29,9 +29,13 @@ class CameraC(ctypes.Structure):
                 ("dvec", c_float * 3)]
 
 
+class RtowOpts(ctypes.Structure):
+    _fields_ = [("fp32", c_int), ("nthreads", c_int), ("argorder", c_int), ("seed_base", c_uint32)]
+
+
 class SceneC(ctypes.Structure):
     _fields_ = [("tris", c_void_p), ("tri_inst", c_void_p), ("ntris", c_uint32), ("things", c_void_p),
-                ("nthings", c_uint32), ("bvh", c_void_p), ("ambient", c_float * 3)]
+                ("nthings", c_uint32), ("bvh", c_void_p), ("ambient", c_float * 3), ("iterative", c_int32)]
 
 
 _lib = None
@@ -73,6 +77,7 @@ def lib():
             "orc_rtow_render": (c_uint64, [c_int, c_int, c_int, c_int, c_int, P]),
             "orc_rtow_rays": (c_uint64, [c_int, c_int, c_int, c_int, c_int, c_uint64]),
             "orc_rtow_scene": (c_uint32, [c_int, P, c_uint32]),
+            "orc_rtow_render_ex": (c_uint64, [POINTER(RtowOpts), c_int, c_int, c_int, c_int, P, P, P, P]),
             "orc_pp_srgb": (None, [P, P, c_int, c_int]),
             "orc_pp_srgb_f64": (None, [P, P, c_int, c_int]),
             "orc_pp_none": (None, [P, P, c_int, c_int]),
@@ -264,7 +269,7 @@ def decidable(res, rays, tris, k=64.0):
 class Scene:
     """Flattened scene for orc_render: soup + per-tri instance + things + BVH."""
 
-    def __init__(self, tris, tri_inst, things, ambient=(.5, .7, 1.)):
+    def __init__(self, tris, tri_inst, things, ambient=(.5, .7, 1.), iterative=False):
         self.tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
         self.tri_inst = np.ascontiguousarray(tri_inst, np.uint32)
         self.things = (Thing * len(things))()
@@ -281,7 +286,7 @@ class Scene:
         self.bvh = BVH(self.tris)
         self.c = SceneC(_p(self.tris).value, _p(self.tri_inst).value, len(self.tris),
                         ctypes.cast(self.things, c_void_p).value, len(things), self.bvh.ptr,
-                        (c_float * 3)(*ambient))
+                        (c_float * 3)(*ambient), 1 if iterative else 0)
 
     def render(self, cam, w, h, spp, depth, y0=0, y1=None, nthreads=8, want_sumsq=False, seed_base=0):
         y1 = h if y1 is None else y1
@@ -320,6 +325,21 @@ def rtow_render(w=1280, h=720, spp=10, depth=50, argorder=1):
     out = np.empty((h, w, 3), np.uint8)
     rays = lib().orc_rtow_render(w, h, spp, depth, argorder, _p(out))
     return out, rays
+
+
+def rtow_render_ex(w, h, spp, depth, fp32=False, nthreads=0, seed_base=0, argorder=1, rgb=False):
+    """The rtow.cxx tracer with options (orc_rtow_render_ex): nthreads 0 =
+    the reference's glibc stream on one thread, else per-pixel drand48
+    streams.  Returns dict(mean=float64[h,w,3] (y = 0 the bottom row),
+    sumsq, rpp=uint32[h,w], rays, rgb=uint8[h,w,3] in PPM order or None)."""
+    o = RtowOpts(1 if fp32 else 0, int(nthreads), int(argorder), int(seed_base))
+    mean = np.zeros((h, w, 3), np.float64)
+    sq = np.zeros((h, w, 3), np.float64)
+    rpp = np.zeros((h, w), np.uint32)
+    out = np.zeros((h, w, 3), np.uint8) if rgb else None
+    rays = lib().orc_rtow_render_ex(ctypes.byref(o), w, h, spp, depth, _p(out) if rgb else None, _p(mean), _p(sq),
+                                    _p(rpp))
+    return dict(mean=mean, sumsq=sq, rpp=rpp, rays=int(rays), rgb=out)
 
 
 def rtow_rays(npix, w=1280, h=720, spp=10, depth=50, argorder=1):

@@ -767,11 +767,19 @@ typedef struct {
  * unrolled into a loop.  The reference multiplies the albedos on the way back
  * up the recursion (col = alb * child); the loop multiplies them into a
  * throughput on the way down, the order the GPU kernel uses (same factors,
- * other association), so that both round alike.  The guide values are the
- * deepest diffuse/metal hit's (the reference keeps the deepest non-zero one).
+ * other association), so that both round alike.  Guide values: the recursive
+ * programs set them after the recursion returns, where still null
+ * (optx/optics_r.cu:143-146, :280-283, util::isnull optx/util_gpu.h:39), so
+ * the deepest diffuse/metal hit's normal and the deepest non-zero albedo win.
+ * With s->iterative the loop follows optx/camera_i.cu:75-95 and
+ * optx/optics_i.cu instead: the hit of the depth-th ray is shaded like any
+ * other and the sample keeps the throughput as its colour, and the guides are
+ * set on the way down where still null (optx/optics_i.cu:98-101): the first
+ * hit's normal and the first non-zero albedo.
  * Returns the colour; *depth = rays traced (payload 5). */
 static f3 trace_path(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
     const orc_scene* s = c->s;
+    const int iter = s->iterative != 0;
     f3 thr = mk(1.f, 1.f, 1.f);
     uint32_t dep = 0;
     for (;;) {
@@ -786,7 +794,7 @@ static f3 trace_path(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
             *depth = dep;
             return mul(thr, sky_f(dir, ld3(s->ambient)));
         }
-        if (c->depth == dep) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :34-42 */
+        if (!iter && c->depth == dep) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :34-42 */
         const orc_thing* th = &s->things[s->tri_inst[h.prim]];
         const float* tri = s->tris + 9 * (size_t)h.prim;
         f3 A = ld3(tri), B = ld3(tri + 3), C = ld3(tri + 6);
@@ -795,21 +803,25 @@ static f3 trace_path(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
         f3 N = unitv(cross(sub(B, A), sub(C, A)));         /* :88 */
         if (dot(dir, N) > 0.f) N = neg(N);                 /* :89-90 */
         f3 nd;
+        if (th->type != ORC_REFRACT) {
+            f3 alb = ld3(th->albedo);
+            if (iter) {                                    /* optx/optics_i.cu:98-101, :173-176 */
+                if (isnull3(c->dgv->normal)) c->dgv->normal = N;
+                if (isnull3(c->dgv->albedo)) c->dgv->albedo = alb;
+            } else {                                       /* optx/optics_r.cu:143-146, :280-283 */
+                c->dgv->normal = N;
+                if (!isnull3(alb)) c->dgv->albedo = alb;
+            }
+        }
         if (th->type == ORC_DIFFUSE) {
             nd = add(N, rnd_on_sphere(c->rng));            /* :108 */
-            f3 alb = ld3(th->albedo);
-            thr = mul(thr, alb);                           /* :135-138 */
-            c->dgv->normal = N;                            /* :143-146 */
-            c->dgv->albedo = alb;
+            thr = mul(thr, ld3(th->albedo));               /* :135-138 */
         } else if (th->type == ORC_REFLECT) {
             f3 d1 = unitv(dir);                            /* :235 */
             f3 refl = reflect_unit(d1, N);                 /* :236 */
             nd = add(refl, scl(th->fuzz, rnd_in_sphere(c->rng)));  /* :238 */
-            f3 alb = ld3(th->albedo);
-            c->dgv->normal = N;                            /* :280-283 */
-            c->dgv->albedo = alb;
             if (!(dot(nd, N) > 0)) { *depth = dep; return mk(0.f, 0.f, 0.f); }   /* :242 */
-            thr = mul(thr, alb);
+            thr = mul(thr, ld3(th->albedo));
         } else {
             f3 d1 = unitv(dir);                            /* :372 */
             float cos_t = fminf(dot(neg(d1), N), 1.f);     /* :373 */
@@ -824,6 +836,7 @@ static f3 trace_path(const tctx* c, f3 ori, f3 dir, uint32_t* depth) {
                 nd = refract_unit(d1, N, ratio);           /* :394-397 */
             /* :425-428 multiply by (1,1,1): exact, left out */
         }
+        if (iter && c->depth == dep) { *depth = dep; return thr; }   /* optx/camera_i.cu:75-95 */
         ori = hit;
         dir = nd;
     }
@@ -1082,27 +1095,6 @@ static d3 rv3(double lo, double hi, int ranged) {
     else { a = ranged ? rrnd2(lo, hi) : rrnd(); b = ranged ? rrnd2(lo, hi) : rrnd(); c = ranged ? rrnd2(lo, hi) : rrnd(); }
     return dk(a, b, c);
 }
-/* v.h:53 */
-static d3 d_in_sphere(void) {
-    for (;;) { d3 v = rv3(-1, 1, 1); if (1 > dsq(v)) return v; }
-}
-/* v.h:59: V( rnd(-1,1), rnd(-1,1), 0 ) */
-static d3 d_in_disk(void) {
-    for (;;) {
-        double a, b;
-        if (g_r2l) { b = rrnd2(-1, 1); a = rrnd2(-1, 1); } else { a = rrnd2(-1, 1); b = rrnd2(-1, 1); }
-        d3 v = dk(a, b, 0);
-        if (1 > dsq(v)) return v;
-    }
-}
-static d3 dreflect(d3 v, d3 n) { return dsub(v, dscl(2 * ddot(v, n), n)); }            /* v.h:61 */
-static d3 drefract(d3 v, d3 n, double ratio) {                                          /* v.h:62 */
-    double theta = fmin(ddot(dneg(v), n), 1.);
-    d3 perpen = dscl(ratio, dadd(v, dscl(theta, n)));
-    d3 parall = dscl(-sqrt(fabs(1. - dsq(perpen))), n);
-    return dadd(perpen, parall);
-}
-
 typedef struct { d3 c; double r; int type; d3 albedo; double fuzz, index; } dsphere;
 
 typedef struct { dsphere* s; int n, cap; } dthings;
@@ -1159,141 +1151,66 @@ uint32_t orc_rtow_scene(int argorder, orc_thing* out, uint32_t cap) {
     return n;
 }
 
-typedef struct { double t; d3 p, normal; int facing; const dsphere* sp; } dbind;   /* thing.h:10-16 */
-
-/* sphere.h:20-48 */
-static int dsphere_hit(const dsphere* s, d3 ro, d3 rd, double tmin, double tmax, dbind* b) {
-    d3 o = dsub(ro, s->c);
-    double a = dsq(rd);
-    double bb = ddot(rd, o);
-    double c = dsq(o) - s->r * s->r;
-    double disc = bb * bb - a * c;
-    if (0 > disc) return 0;
-    double x = sqrt(disc);
-    double t = (-bb - x) / a;
-    if (tmin > t || t > tmax) {
-        t = (-bb + x) / a;
-        if (tmin > t || t > tmax) return 0;
-    }
-    b->t = t;
-    b->p = dadd(ro, dscl(b->t, rd));                   /* ray.h:14 */
-    d3 outward = ddiv(dsub(b->p, s->c), s->r);
-    b->facing = 0 > ddot(rd, outward);
-    b->normal = b->facing ? outward : dneg(outward);
-    b->sp = s;
-    return 1;
+/* Random draws of the tracer (util.h:12): glibc rand() as the reference
+ * draws them, or a pixel's own drand48 stream (a = 0x5DEECE66D, c = 0xB mod
+ * 2^48: the value erand48 returns, never 1) for the multi-threaded runs */
+typedef struct { int r2l; int glibc; uint64_t st; uint64_t rays; } rtrng;
+static inline double rt_glibc_draw_d(void) { return rand() / (RAND_MAX + 1.); }
+/* FP32: the top 24 of rand()'s 31 bits, so that the float stays below 1 */
+static inline float rt_glibc_draw_f(void) { return (float)(rand() >> 7) * 0x1p-24f; }
+static inline void rt_stream_next(uint64_t* st) { *st = (0x5DEECE66Dull * *st + 0xBull) & 0xFFFFFFFFFFFFull; }
+static inline double rt_stream_draw_d(uint64_t* st) { rt_stream_next(st); return (double)*st * 0x1p-48; }
+static inline float rt_stream_draw_f(uint64_t* st) { rt_stream_next(st); return (float)(*st >> 24) * 0x1p-24f; }
+static inline void rt_stream_seed(uint64_t* st, uint32_t seed) {
+    *st = seed;
+    for (int i = 0; i < 16; i++) rt_stream_next(st);
 }
 
-/* things.h:22-36 */
-static int dthings_hit(const dthings* t, d3 ro, d3 rd, double tmin, double tmax, dbind* b) {
-    dbind buf;
-    int shot = 0;
-    double tact = tmax;
-    for (int i = 0; i < t->n; i++)
-        if (dsphere_hit(&t->s[i], ro, rd, tmin, tact, &buf)) { shot = 1; tact = buf.t; *b = buf; }
-    return shot;
-}
+#define RT_R double
+#define RT_S _d
+#define RT_SQRT sqrt
+#define RT_FABS fabs
+#define RT_FMIN fmin
+#define RT_POW pow
+#define RT_TAN tan
+#include "rtow.inc"
+#undef RT_R
+#undef RT_S
+#undef RT_SQRT
+#undef RT_FABS
+#undef RT_FMIN
+#undef RT_POW
+#undef RT_TAN
 
-static uint64_t g_rays;
-
-/* optics.h:15-68: returns 1 if sprayed */
-static int dspray(d3 rd, const dbind* b, d3* att, d3* sro, d3* srd) {
-    const dsphere* s = b->sp;
-    if (s->type == ORC_DIFFUSE) {
-        d3 dir = dadd(b->normal, dunit(d_in_sphere()));
-        const double kNear0 = 1e-8f;
-        if (fabs(dir.x) < kNear0 && fabs(dir.y) < kNear0 && fabs(dir.z) < kNear0) dir = b->normal;
-        *sro = b->p; *srd = dir; *att = s->albedo;
-        return 1;
-    } else if (s->type == ORC_REFLECT) {
-        d3 r = dreflect(dunit(rd), b->normal);
-        *sro = b->p; *srd = dadd(r, dscl(s->fuzz, d_in_sphere()));
-        *att = s->albedo;
-        return ddot(*srd, b->normal) > 0;
-    } else {
-        d3 dir, d1 = dunit(rd);
-        double cos_t = fmin(ddot(dneg(d1), b->normal), 1.);
-        double sin_t = sqrt(1. - cos_t * cos_t);
-        double ratio = b->facing ? 1. / s->index : s->index;
-        int cannot = ratio * sin_t > 1.;
-        double r0 = (1 - ratio) / (1 + ratio); r0 = r0 * r0;
-        if (cannot || r0 + (1 - r0) * pow((1 - cos_t), 5) > rrnd()) dir = dreflect(d1, b->normal);
-        else dir = drefract(d1, b->normal, ratio);
-        *sro = b->p; *srd = dir; *att = dk(1., 1., 1.);
-        return 1;
-    }
-}
-
-/* rtow.cxx:34-49 */
-static d3 dtrace(const dthings* t, d3 ro, d3 rd, int depth) {
-    dbind b;
-    memset(&b, 0, sizeof(b));
-    const double kAcne0 = 1e-3f;
-    g_rays++;
-    if (dthings_hit(t, ro, rd, kAcne0, INFINITY, &b)) {
-        d3 sro, srd, att;
-        if (depth > 0 && dspray(rd, &b, &att, &sro, &srd)) return dmul(att, dtrace(t, sro, srd, depth - 1));
-        return dk(0, 0, 0);
-    }
-    d3 unit = dunit(rd);
-    double tt = .5 * (unit.y + 1.);
-    return dadd(dscl(1. - tt, dk(1., 1., 1.)), dscl(tt, dk(.5, .7, 1.)));
-}
-
-static inline double dclamp(double x, double lo, double hi) { return lo > x ? lo : x > hi ? hi : x; }
-
-/* the first npix pixels in the reference's output order (rows from the top);
-   rgb may be NULL (ray count only) */
-static uint64_t rtow_render_n(int w, int h, int spp, int depth, int argorder, uint8_t* rgb, uint64_t npix) {
-    g_r2l = argorder;
-    srand(1);
-    dthings things = rtow_scene();
-    /* rtow.cxx:85-94 and camera.h:10-23 */
-    double aspratio = 16. / 9.;
-    d3 eye = dk(13., 2., 3.), pat = dk(0., 0., 0.), vup = dk(0., 1., 0.);
-    double aperture = .1, fostance = 10.;
-    const double kPi = 3.141592653589793238;
-    d3 cw = dunit(dsub(eye, pat));
-    d3 cu = dunit(dcross(vup, cw));
-    d3 cv = dcross(cw, cu);
-    double hh = 2. * tan(.5 * 20. * kPi / 180.);
-    double ww = hh * aspratio;
-    d3 hvec = dscl(fostance * hh / 2., cv);
-    d3 wvec = dscl(fostance * ww / 2., cu);
-    d3 dvec = dscl(fostance, cw);
-    g_rays = 0;
-    size_t o = 0;
-    uint64_t done = 0;
-    for (int y = h - 1; y >= 0 && done < npix; --y) {  /* rtow.cxx:105-121 */
-        for (int x = 0; x < w && done < npix; ++x, ++done) {
-            d3 color = dk(0, 0, 0);
-            for (int k = 0; k < spp; ++k) {
-                double s = 2. * (x + rrnd()) / (w - 1) - 1.;
-                double t = 2. * (y + rrnd()) / (h - 1) - 1.;
-                d3 r = dscl(aperture / 2., d_in_disk());               /* camera.h:27-30 */
-                d3 off = dadd(dscl(r.x, cu), dscl(r.y, cv));
-                d3 ro = dadd(eye, off);
-                d3 rd = dsub(dsub(dadd(dscl(s, wvec), dscl(t, hvec)), dvec), off);
-                color = dadd(color, dtrace(&things, ro, rd, depth));
-            }
-            if (!rgb) continue;
-            d3 c = ddiv(color, spp);                                  /* rtow.cxx:119 */
-            double r = sqrt(c.x), g = sqrt(c.y), b = sqrt(c.z);       /* rtow.cxx:13 */
-            rgb[o++] = (uint8_t)(int)(256 * dclamp(r, 0, .999));
-            rgb[o++] = (uint8_t)(int)(256 * dclamp(g, 0, .999));
-            rgb[o++] = (uint8_t)(int)(256 * dclamp(b, 0, .999));
-        }
-    }
-    free(things.s);
-    return g_rays;
-}
+#define RT_R float
+#define RT_S _f
+#define RT_SQRT sqrtf
+#define RT_FABS fabsf
+#define RT_FMIN fminf
+#define RT_POW(x, n) powf((x), (float)(n))
+#define RT_TAN tanf
+#include "rtow.inc"
+#undef RT_R
+#undef RT_S
+#undef RT_SQRT
+#undef RT_FABS
+#undef RT_FMIN
+#undef RT_POW
+#undef RT_TAN
 
 uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t* rgb) {
-    return rtow_render_n(w, h, spp, depth, argorder, rgb, (uint64_t)w * (uint64_t)h);
+    return rtow_run_d(w, h, spp, depth, argorder, 0, 0, (uint64_t)w * (uint64_t)h, rgb, NULL, NULL, NULL);
 }
 
 uint64_t orc_rtow_rays(int w, int h, int spp, int depth, int argorder, uint64_t npix) {
-    return rtow_render_n(w, h, spp, depth, argorder, NULL, npix);
+    return rtow_run_d(w, h, spp, depth, argorder, 0, 0, npix, NULL, NULL, NULL, NULL);
+}
+
+uint64_t orc_rtow_render_ex(const orc_rtow_opts* o, int w, int h, int spp, int depth, uint8_t* rgb, double* mean,
+                            double* sumsq, uint32_t* rpp) {
+    const uint64_t all = (uint64_t)w * (uint64_t)h;
+    if (o->fp32) return rtow_run_f(w, h, spp, depth, o->argorder, o->nthreads, o->seed_base, all, rgb, mean, sumsq, rpp);
+    return rtow_run_d(w, h, spp, depth, o->argorder, o->nthreads, o->seed_base, all, rgb, mean, sumsq, rpp);
 }
 
 /* ========================================================================= */

@@ -21,11 +21,25 @@
  *   orc_watertight_f32   == FP64 brute force away from the edge epsilon; closed
  *                        mesh leaks no ray through shared edges/vertices.
  *   orc_rtow_render      bit-exact against the reference rtow binary built from
- *                        rtow.cxx (oracle/ref.mk) on its own hard-coded image.
- *   orc_render           its per-material steps are the ones rtow.cxx pins (the
- *                        GPU deviations Q1-Q7 are each a KAT), and it is checked
- *                        statistically against orc_rtow_render-style analytic
- *                        scenes; parity of the whole mesh render is "statistical".
+ *                        rtow.cxx (oracle/ref.mk) on its own hard-coded image;
+ *                        orc_rtow_render_ex runs the same tracer (FP64 on the
+ *                        glibc stream is that image), or in FP32, or on
+ *                        per-pixel streams over threads.
+ *   orc_render           (tests/test_oracle_closed_forms.py) images the
+ *                        mathematics fixes, for both program sets: the sky of
+ *                        an empty scene with rpp == spp, black (recursive) or
+ *                        the throughput (iterative) at the depth limit, the
+ *                        cosine-weighted sky mean a(1/6 + 5/6 amb) over a
+ *                        Lambertian ground, the mirror direction of a fuzz-0
+ *                        metal, the absorbed share of a fuzzy metal, Schlick's
+ *                        reflected share with 1/ior on entering a glass slab,
+ *                        the guide values of both program sets;
+ *                        (tests/test_oracle_vs_rtow.py) statistical agreement
+ *                        with orc_rtow_render on rtow.cxx's own scene,
+ *                        tessellated, at lp.depth 16 and 3: channel means
+ *                        within 3 sigma plus the measured tessellation term,
+ *                        25x25 tiles within chi^2 < 1.6 per tile and 5 sigma,
+ *                        rays per sample.
  *   orc_pp_srgb          IEC 61966-2-1 curve evaluated in FP64.
  *   orc_render_frozen    equals orc_render at the scene's own materials (to FP32
  *                        rounding); its albedo dependence is a product (checked
@@ -125,6 +139,7 @@ typedef struct {
     uint32_t         nthings;
     const orc_bvh*   bvh;       /* over tris */
     float            ambient[3];/* miss record, optx/rtwo.cxx:110-111 */
+    int32_t          iterative; /* 0: optx/camera_r.cu + optics_r.cu; 1: optx/camera_i.cu + optics_i.cu */
 } orc_scene;
 
 /* Renders rows [y0,y1) of a w x h image.  Outputs are full-size buffers
@@ -158,6 +173,21 @@ uint64_t orc_rtow_render(int w, int h, int spp, int depth, int argorder, uint8_t
 uint64_t orc_rtow_rays(int w, int h, int spp, int depth, int argorder, uint64_t npix);
 /* the rtow scene (rtow.cxx:51-80) as things, same argorder convention */
 uint32_t orc_rtow_scene(int argorder, orc_thing* out, uint32_t cap);
+/* The same tracer with options.  fp32: 0 = FP64 (the reference's double), 1 =
+ * every operation in float.  nthreads: 0 = one thread drawing from glibc
+ * rand() after srand(1) in the reference's pixel order (fp32 = 0 gives the
+ * reference's image byte for byte); >= 1 = that many threads, each pixel
+ * drawing from its own drand48 stream seeded with pix + seed_base (pix =
+ * w*y + x, y = 0 the bottom row).  Pixel mapping, camera, scene and depth
+ * semantics are rtow.cxx's: s = 2(x+u)/(w-1)-1, trace(ray, depth) allows
+ * depth scatters (depth+1 rays).  Outputs (any may be NULL): rgb w*h*3
+ * bytes in PPM order (top row first, gamma 2), mean 3*w*h doubles (the
+ * pixel's colour/spp before gamma, pix order), sumsq 3*w*h (sum over
+ * samples of the squared sample colour), rpp w*h (rays per pixel).
+ * Returns the rays traced. */
+typedef struct { int fp32; int nthreads; int argorder; uint32_t seed_base; } orc_rtow_opts;
+uint64_t orc_rtow_render_ex(const orc_rtow_opts* o, int w, int h, int spp, int depth, uint8_t* rgb, double* mean,
+                            double* sumsq, uint32_t* rpp);
 
 /* ---- Scatter building blocks used by orc_render (for known-answer tests) */
 void  orc_reflect(const float d[3], const float n[3], float out[3]);              /* of unit(d) */

@@ -118,6 +118,8 @@ SIGNATURES = {
     "rtx_scene_stats_get": (c_int, [c_void_p, c_uint64, POINTER(SceneStatsC)]),
     "rtx_scene_triangles": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_set_miss_color": (c_int, [c_void_p, Float3]),
+    "rtx_set_semantics": (c_int, [c_void_p, c_int]),
+    "rtx_get_semantics": (c_int, [c_void_p, POINTER(c_int)]),
     "rtx_tuning_get": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_tuning_set": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_launch": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),

@@ -365,6 +365,19 @@ class Scene:
     def set_miss_color(self, ambient):
         call("rtx_set_miss_color", self.ctx, _f3(ambient))
 
+    RECURSIVE, ITERATIVE = 0, 1
+
+    def semantics(self, mode=None):
+        """Reads (and with an argument sets) which program set launches follow:
+        Scene.RECURSIVE (optx/camera_r.cu + optx/optics_r.cu, the default
+        here) or Scene.ITERATIVE (optx/camera_i.cu + optx/optics_i.cu, the
+        reference's default build, optx/rtwo.cxx:44-50)."""
+        if mode is not None:
+            call("rtx_set_semantics", self.ctx, int(mode))
+        m = c_int(0)
+        call("rtx_get_semantics", self.ctx, byref(m))
+        return m.value
+
     def trace(self, rays, handle=None, stream=None):
         """Closest hits for rays float32[n,8] (ori, tmin, dir, tmax) -> structured hits."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

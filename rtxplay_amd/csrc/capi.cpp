@@ -435,6 +435,20 @@ rtx_status rtx_set_miss_color(rtx_ctx* c, rtx_float3 ambient) {
     return RTX_SUCCESS;
 }
 
+rtx_status rtx_set_semantics(rtx_ctx* c, int mode) {
+    if (!c) return fail(RTX_ERROR_INVALID_VALUE, "rtx_set_semantics: ctx is NULL");
+    if (mode != RTX_SEMANTICS_RECURSIVE && mode != RTX_SEMANTICS_ITERATIVE)
+        return fail(RTX_ERROR_INVALID_VALUE, "rtx_set_semantics: unknown mode");
+    c->render_ls.iterative = mode == RTX_SEMANTICS_ITERATIVE ? 1u : 0u;
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_get_semantics(rtx_ctx* c, int* mode) {
+    if (!c || !mode) return fail(RTX_ERROR_INVALID_VALUE, "rtx_get_semantics: NULL argument");
+    *mode = c->render_ls.iterative ? RTX_SEMANTICS_ITERATIVE : RTX_SEMANTICS_RECURSIVE;
+    return RTX_SUCCESS;
+}
+
 // Per instance: the flatten record (transform, mesh, first global triangle)
 // and the material record of the scene (2 float4); N = triangles of all
 // instances.
@@ -1061,7 +1075,8 @@ struct rtx_batch {
 
 static bool same_scratch(const rtx::LaunchScratch& a, const rtx::LaunchScratch& b) {
     return a.counter == b.counter && a.gstack == b.gstack && a.gstride == b.gstride && a.grid == b.grid &&
-           a.shade_batch == b.shade_batch && a.inner_frac == b.inner_frac && a.stack_lds == b.stack_lds;
+           a.shade_batch == b.shade_batch && a.inner_frac == b.inner_frac && a.stack_lds == b.stack_lds &&
+           a.iterative == b.iterative;
 }
 
 extern "C" {
@@ -1245,14 +1260,17 @@ rtx_status rtx_launch_adjoint(rtx_ctx* c, const rtx_lp_general* lp, const rtx_fl
     HIP_OK(hipSetDevice(c->device));
     if (rtx_status st = ensure_gstack(c->render_ls, b->stack_need)) return st;
     if (c->tape_depth < lp->depth) {
-        // (depth - 1) path records of 14 words for every lane of the grid
+        // path records of 14 words for every lane of the grid: a path records
+        // each hit it shades, at most depth - 1 under the recursive programs
+        // (the depth-th ray's hit is black unshaded) and depth under the
+        // iterative ones (that hit is shaded too); sized for depth
         if (c->tape) {
             HIP_OK(hipDeviceSynchronize());
             HIP_OK(hipFree(c->tape));
             c->tape = nullptr;
             c->tape_depth = 0;
         }
-        const size_t words = (size_t)(lp->depth > 1 ? lp->depth - 1 : 1) * rtx::kAdjointTapeWords * c->render_ls.gstride;
+        const size_t words = (size_t)lp->depth * rtx::kAdjointTapeWords * c->render_ls.gstride;
         HIP_OK(hipMalloc((void**)&c->tape, words * sizeof(uint32_t)));
         c->tape_depth = lp->depth;
     }

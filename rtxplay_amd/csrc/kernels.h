@@ -27,6 +27,7 @@ struct LaunchScratch {
     uint32_t* tile_cost = nullptr;   // render: rays per tile of the one-sample probe (device)
     uint32_t* tile_order = nullptr;  // render: the tiles in descending probe cost (device)
     uint32_t tile_cap = 0;           // tiles the two arrays hold
+    uint32_t iterative = 0;          // render: 1 = the iterative programs' semantics, 0 = the recursive ones
 };
 
 // Plain frames of at least kOrderMinSpp samples per pixel and at least
@@ -77,7 +78,8 @@ hipError_t launch_render_batch(const Job* d_jobs, const Job& first, uint32_t njo
                                const LaunchScratch& ls, uint32_t tile_w, hipStream_t s);
 // adjoint of a frame: dparams (float4 per instance: d albedo rgb, d fuzz) =
 // dL/d params given dL/d rawRGB (lp.rawRGB holds the forward image); tape:
-// (depth - 1) x kAdjointTapeWords words per lane, lane stride tape_stride
+// depth x kAdjointTapeWords words per lane (one record per shaded hit: at
+// most depth - 1 recursive, depth iterative), lane stride tape_stride
 constexpr uint32_t kAdjointTapeWords = 14;
 constexpr uint32_t kAdjointLdsInsts = 2048;   // instances whose gradients a block sums in LDS first
 hipError_t launch_adjoint(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, const LaunchScratch& ls,

@@ -62,6 +62,9 @@ struct RenderArgs {
     const uint32_t* tile_order;
     // probe launch: rays per tile (one sample per pixel, nothing else written)
     uint32_t* tile_cost;
+    // 1: the iterative programs' semantics (optx/camera_i.cu, optx/optics_i.cu),
+    // 0: the recursive ones (optx/camera_r.cu, optx/optics_r.cu)
+    uint32_t iterative;
 };
 
 // path record of the adjoint: meta (instance | type << 24 | flags), the
@@ -74,6 +77,8 @@ constexpr uint32_t kTapeCosClamp = 1u << 27;  // dielectric: cos theta clamped t
 
 __device__ __forceinline__ float3 ld(const rtx_float3& v) { return f3(v.x, v.y, v.z); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return lo > x ? lo : x > hi ? hi : x; }
+// util::isnull, optx/util_gpu.h:39
+__device__ __forceinline__ bool isnull(float3 v) { return v.x == 0.f && v.y == 0.f && v.z == 0.f; }
 __device__ __forceinline__ void st(rtx_float3* p, float3 v) {
     p->x = v.x;
     p->y = v.y;
@@ -234,6 +239,49 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         const float3 d = tv.r.d;
         bool end = false;
         float3 col = f3(0.f, 0.f, 0.f);
+        // adjoint of a finished path whose radiance is (product of the
+        // albedos) * S0: walk the path records backwards with S = the product
+        // of the factors after the vertex (S0 included) and lam = dL/d(outgoing
+        // direction), which flows back through metal and dielectric vertices
+        // up to the last diffuse one (chain: the last vertex's outgoing
+        // direction matters)
+        auto backprop = [&](float3 S, float3 lam, bool chain) {
+            for (int j = (int)kv - 1; j >= 0; --j) {
+                const uint32_t meta = tget(j, 0);
+                const uint32_t inst = meta & 0xFFFFFFu;
+                const int type = (int)((meta >> 24) & 3u);
+                if (type != RTX_OPTICS_REFRACT) {
+                    const float3 P = tget3(j, 1);
+                    const float3 alb = xyz(ld4<BATCH>(sc.insts + 2 * inst));
+                    const float3 ga = delta * P * S;
+                    grad(inst, 0, ga.x);
+                    grad(inst, 1, ga.y);
+                    grad(inst, 2, ga.z);
+                    S = S * alb;
+                    if (type == RTX_OPTICS_DIFFUSE) chain = false;
+                }
+                if (!chain) continue;
+                const float3 u = tget3(j, 4);
+                const float iln = tgetf(j, 7);
+                const float3 N = tget3(j, 8);
+                float3 lu;                                  // dL/d(unit incoming direction)
+                if (type == RTX_OPTICS_REFLECT) {
+                    grad(inst, 3, dot(lam, tget3(j, 11)));
+                    lu = lam - (2.f * dot(lam, N)) * N;
+                } else if (!(meta & kTapeRefract)) {
+                    lu = lam - (2.f * dot(lam, N)) * N;
+                } else {
+                    const float ratio = tgetf(j, 11);
+                    const float c = (meta & kTapeCosClamp) ? 1.f : -dot(u, N);
+                    const float3 pp = ratio * (u + c * N);
+                    const float q = 1.f - dot(pp, pp);
+                    const float sq = sqrtf(fabsf(q));
+                    const float3 lpp = lam + ((q < 0.f ? -1.f : 1.f) / sq * dot(N, lam)) * pp;
+                    lu = (meta & kTapeCosClamp) ? ratio * lpp : ratio * (lpp - dot(N, lpp) * N);
+                }
+                lam = iln * (lu - dot(u, lu) * u);
+            }
+        };
         if (tv.h.tri == kNoHit) {
             // __miss__ambient, optx/camera_r.cu:135-146
             const float3 unit = unitv(d);
@@ -242,55 +290,14 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             col = thr * sky;
             end = true;
             if (ADJ) {
-                // adjoint of the escaped path: radiance = (prod of albedos) * sky(unit(d)).
-                // Walk the path records backwards with S = the product of the
-                // factors after the vertex (sky included) and lam = dL/d(outgoing
-                // direction), which flows back through metal and dielectric
-                // vertices up to the last diffuse one.
+                // radiance = (prod of albedos) * sky(unit(d)); dL/d d through the sky gradient
                 const float3 amb = J->ambient;
                 const float gam = .5f * (delta.x * thr.x * (amb.x - 1.f) + delta.y * thr.y * (amb.y - 1.f) +
                                          delta.z * thr.z * (amb.z - 1.f));
                 const float il = 1.f / sqrtf(dot(d, d));
-                float3 lam = (gam * il) * (f3(0.f, 1.f, 0.f) - unit.y * unit);
-                float3 S = sky;
-                bool chain = true;
-                for (int j = (int)kv - 1; j >= 0; --j) {
-                    const uint32_t meta = tget(j, 0);
-                    const uint32_t inst = meta & 0xFFFFFFu;
-                    const int type = (int)((meta >> 24) & 3u);
-                    if (type != RTX_OPTICS_REFRACT) {
-                        const float3 P = tget3(j, 1);
-                        const float3 alb = xyz(ld4<BATCH>(sc.insts + 2 * inst));
-                        const float3 gA = delta * P * S;
-                        grad(inst, 0, gA.x);
-                        grad(inst, 1, gA.y);
-                        grad(inst, 2, gA.z);
-                        S = S * alb;
-                        if (type == RTX_OPTICS_DIFFUSE) chain = false;
-                    }
-                    if (!chain) continue;
-                    const float3 u = tget3(j, 4);
-                    const float iln = tgetf(j, 7);
-                    const float3 N = tget3(j, 8);
-                    float3 lu;                                  // dL/d(unit incoming direction)
-                    if (type == RTX_OPTICS_REFLECT) {
-                        grad(inst, 3, dot(lam, tget3(j, 11)));
-                        lu = lam - (2.f * dot(lam, N)) * N;
-                    } else if (!(meta & kTapeRefract)) {
-                        lu = lam - (2.f * dot(lam, N)) * N;
-                    } else {
-                        const float ratio = tgetf(j, 11);
-                        const float c = (meta & kTapeCosClamp) ? 1.f : -dot(u, N);
-                        const float3 pp = ratio * (u + c * N);
-                        const float q = 1.f - dot(pp, pp);
-                        const float sq = sqrtf(fabsf(q));
-                        const float3 lpp = lam + ((q < 0.f ? -1.f : 1.f) / sq * dot(N, lam)) * pp;
-                        lu = (meta & kTapeCosClamp) ? ratio * lpp : ratio * (lpp - dot(N, lpp) * N);
-                    }
-                    lam = iln * (lu - dot(u, lu) * u);
-                }
+                backprop(sky, (gam * il) * (f3(0.f, 1.f, 0.f) - unit.y * unit), true);
             }
-        } else if (k == maxd) {
+        } else if (k == maxd && !a.iterative) {
             end = true;                                    // optx/optics_r.cu:34-42
         } else {
             // hit reconstruction, optx/optics_r.cu:45-90
@@ -324,21 +331,31 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
             if (diff || metal) rs = rnd_in_sphere(rng);
             const float3 un = unitv(diff ? rs : d);
             const float3 refl = un - (2.f * dot(un, N)) * N;   // optx/optics_r.cu:236, :392
-            if (diff) {
-                // optx/optics_r.cu:108, :135-146
-                nd = N + un;
+            if (diff || metal) {
+                // denoiser guide values.  Recursive programs set them after
+                // the recursion returns, where still null (optx/optics_r.cu:
+                // 143-146, :280-283): the deepest diffuse or metal hit's
+                // normal and the deepest non-zero albedo.  Iterative ones set
+                // them on the way down (optx/optics_i.cu:98-101, :173-176):
+                // the first hit's normal and the first non-zero albedo.
                 const float3 alb = xyz(m0);
-                thr = thr * alb;
-                gN = N;
-                gA = alb;
+                if (a.iterative) {
+                    if (isnull(gN)) gN = N;
+                    if (isnull(gA)) gA = alb;
+                } else {
+                    gN = N;
+                    if (!isnull(alb)) gA = alb;
+                }
+            }
+            if (diff) {
+                // optx/optics_r.cu:108, :135-138
+                nd = N + un;
+                thr = thr * xyz(m0);
             } else if (metal) {
-                // optx/optics_r.cu:235-283
+                // optx/optics_r.cu:235-279
                 nd = refl + m0.w * rs;
                 if (ADJ) tput3(kv, 11, rs);
-                const float3 alb = xyz(m0);
-                gN = N;
-                gA = alb;
-                if (dot(nd, N) > 0.f) thr = thr * alb;
+                if (dot(nd, N) > 0.f) thr = thr * xyz(m0);
                 else end = true;                           // absorbed: black
             } else {
                 // optx/optics_r.cu:372-398
@@ -373,7 +390,13 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
                 }
                 kv++;
             }
-            if (!end) {
+            if (!end && k == maxd) {
+                // iterative semantics at the last ray (optx/camera_i.cu:75-95):
+                // the loop ends and the sample keeps the throughput as its colour
+                end = true;
+                col = thr;
+                if (ADJ) backprop(f3(1.f, 1.f, 1.f), f3(0.f, 0.f, 0.f), false);
+            } else if (!end) {
                 ro = hp;
                 rdv = nd;
                 cont = true;
@@ -617,6 +640,7 @@ RenderArgs base_args(const LaunchScratch& ls) {
     a.batch = ls.shade_batch ? ls.shade_batch : 40u;
     a.inner_frac = ls.inner_frac ? ls.inner_frac : 36u;
     a.tile_log2w = log2u(ls.tile_w ? ls.tile_w : 8u);
+    a.iterative = ls.iterative;
     return a;
 }
 

@@ -28,15 +28,19 @@ def _check_runtime():
 class _Render(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, r):
-        img = r.forward(params)
+        p = r.prepare(params)
+        img = r.render(p)
         ctx.r = r
-        ctx.save_for_backward(img)
+        ctx.save_for_backward(img, p)
         return img
 
     @staticmethod
     def backward(ctx, grad_img):
-        img, = ctx.saved_tensors
-        return ctx.r.adjoint(img, grad_img), None
+        img, p = ctx.saved_tensors
+        # the adjoint replays the forward's paths with the scene's materials:
+        # write this forward's parameters back first (another forward may
+        # have set the scene's materials since)
+        return ctx.r.adjoint(img, grad_img, p, prepared=True), None
 
 
 class DiffRender:
@@ -59,24 +63,42 @@ class DiffRender:
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def forward(self, params):
-        p = params.detach().to(self.device, torch.float32).contiguous()
+    def prepare(self, params):
+        """params as a contiguous float32 [things, 4] tensor on the device (a
+        copy: later in-place changes of params do not reach it)."""
+        p = params.detach().to(self.device, torch.float32).clone().contiguous()
         if p.shape != (self.n, 4):
             raise ValueError("params must be [%d, 4]" % self.n)
-        s = self._stream()
+        return p
+
+    def _set_materials(self, p, s):
         call("rtx_scene_set_materials", self.scene.ctx, ctypes.c_uint64(self.lp.is_handle),
              ctypes.c_void_p(p.data_ptr()), s)
+
+    def forward(self, params):
+        """The image for params (no autograd record; __call__ records one)."""
+        return self.render(self.prepare(params))
+
+    def render(self, p):
+        """The image for prepared parameters p (see prepare)."""
+        s = self._stream()
+        self._set_materials(p, s)
         img = torch.empty(self.shape, dtype=torch.float32, device=self.device)
         self.lp.rawRGB = img.data_ptr()
         call("rtx_launch", self.scene.ctx, ctypes.byref(self.lp), 0, s)
         return img
 
-    def adjoint(self, img, grad_img):
+    def adjoint(self, img, grad_img, params=None, prepared=False):
+        """dL/d params for the forward image img rendered with params (the
+        scene's current materials when None)."""
+        s = self._stream()
+        if params is not None:
+            self._set_materials(params if prepared else self.prepare(params), s)
         g = grad_img.detach().to(self.device, torch.float32).contiguous()
         d = torch.empty((self.n, 4), dtype=torch.float32, device=self.device)
         self.lp.rawRGB = img.data_ptr()
         call("rtx_launch_adjoint", self.scene.ctx, ctypes.byref(self.lp), ctypes.c_void_p(g.data_ptr()),
-             ctypes.c_void_p(d.data_ptr()), self._stream())
+             ctypes.c_void_p(d.data_ptr()), s)
         return d
 
     def __call__(self, params):

@@ -159,6 +159,32 @@ def test_albedo_gradient_matches_finite_differences(c5, rtx):
     say("worst albedo rel err %.2e" % worst)
 
 
+def test_albedo_gradient_iterative_semantics(rtx):
+    """Under the iterative programs' semantics a sample that reaches the
+    depth limit keeps its throughput (a product of albedos): at depth 3 most
+    samples do, and the adjoint's albedo gradients match central differences
+    of the GPU render."""
+    f = Frame(rtx, 4, 160, 90, 16, 3)
+    f.sc.semantics(f.sc.ITERATIVE)
+    try:
+        mt = _target_params(f.m0, f.types)
+        T = f.render(mt).astype(np.float64)
+        I = f.render(f.m0)
+        g = _adjoint(f, I, T)
+        cand = sorted(((abs(g[k, c]), k, c) for k in range(len(g)) for c in range(3)), reverse=True)[:8]
+        for _, k, c in cand:
+            mp, mm = f.m0.copy(), f.m0.copy()
+            mp[k, c] = np.float32(f.m0[k, c] + H_FD)
+            mm[k, c] = np.float32(f.m0[k, c] - H_FD)
+            fd = (_loss(f.render(mp), T) - _loss(f.render(mm), T)) / (float(mp[k, c]) - float(mm[k, c]))
+            rel = abs(g[k, c] - fd) / abs(fd)
+            say("iterative albedo", k, c, "adjoint %.6e fd %.6e rel %.2e" % (g[k, c], fd, rel))
+            assert rel <= REL, (k, c, g[k, c], fd)
+    finally:
+        f.sc.semantics(f.sc.RECURSIVE)
+        f.close()
+
+
 @pytest.fixture(scope="module")
 def small(rtx, orc):
     f = Frame(rtx, 4, 160, 90, 16, 8)
@@ -276,4 +302,33 @@ def test_torch_harness(small, rtx):
         losses.append(float(loss.detach()))
     say("adam losses", ["%.2f" % x for x in losses[::6]], "->", "%.2f" % losses[-1])
     assert losses[-1] < 0.5 * losses[0]
+    small.sc.set_materials(small.m0)
+
+
+def test_torch_harness_two_forwards_before_backward(small, rtx):
+    """loss = f(p1) + f(p2): the second forward leaves p2 in the scene, and
+    the first forward's backward must still replay its paths with p1; the
+    gradients equal those of the two losses taken one at a time."""
+    import torch
+    from rtxplay_amd.diff import DiffRender
+    r = DiffRender(small.sc, small.lp)
+    T = torch.tensor(small.T, dtype=torch.float32, device="cuda")
+    p1 = torch.tensor(small.m0, device="cuda")
+    p2 = torch.tensor(small.mt, device="cuda")
+    grads = []
+    for ps in ((p1,), (p2,)):
+        q = ps[0].clone().requires_grad_(True)
+        ((r(q) - T) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        grads.append(q.grad.detach().cpu().numpy().astype(np.float64))
+    a = p1.clone().requires_grad_(True)
+    b = p2.clone().requires_grad_(True)
+    (((r(a) - T) ** 2).sum() + ((r(b) - T) ** 2).sum()).backward()
+    torch.cuda.synchronize()
+    ga, gb = a.grad.detach().cpu().numpy(), b.grad.detach().cpu().numpy()
+    scale = max(np.abs(grads[0]).max(), np.abs(grads[1]).max())
+    say("two forwards: max |dg| %.2e of %.2e" % (max(np.abs(ga - grads[0]).max(), np.abs(gb - grads[1]).max()), scale))
+    assert np.abs(ga - grads[0]).max() <= 1e-4 * scale
+    assert np.abs(gb - grads[1]).max() <= 1e-4 * scale
+    assert np.abs(grads[0] - grads[1]).max() > 1e-2 * scale      # the two parameter sets differ
     small.sc.set_materials(small.m0)

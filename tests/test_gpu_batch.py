@@ -85,6 +85,33 @@ def test_batch_graph_replays(rtx, small_batch):
     b.free()
 
 
+def test_batch_iterative_semantics_equals_plain_launches(rtx):
+    """Depth 3 under the iterative programs' semantics (set on every frame's
+    context; the batch takes its first context's): each frame of the graph
+    launch equals its plain launch bit for bit."""
+    from rtxplay_amd.hip import Stream
+    from rtxplay_amd.scene import BatchFrames
+    bf = BatchFrames(_frames())
+    try:
+        for sc, lp in zip(bf.scenes, bf.lps):
+            sc.semantics(sc.ITERATIVE)
+            lp.depth = 3
+        for l in bf.launchers:
+            l.ignite()
+        ref = _outputs(bf)
+        _clear(bf)
+        b = rtx.Batch(bf.scenes[0], bf.lps, use_graph=True)
+        st = Stream()
+        say("iterative batch, depth 3")
+        b.launch(st)
+        st.synchronize()
+        for i, (g, r) in enumerate(zip(_outputs(bf), ref)):
+            assert _same(g, r), "frame %d differs" % i
+        b.free()
+    finally:
+        bf.close()
+
+
 @pytest.mark.parametrize("tuning", [dict(tile_w=16), dict(tile_w=2, stack_lds=16), dict(shade_batch=8, inner_frac=48),
                                     dict(blocks_per_cu=1, tile_w=64), dict(shade_batch=64, inner_frac=1)])
 def test_tuning_does_not_change_pixels(rtx, small_batch, tuning):

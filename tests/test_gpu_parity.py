@@ -260,6 +260,32 @@ def test_render_matches_oracle(rtx, orc, level4):
     L.free()
 
 
+def test_render_matches_oracle_iterative(rtx, orc, level4):
+    """The iterative programs' semantics (rtx_set_semantics, optx/camera_i.cu
+    + optx/optics_i.cu) at depth 4, where many samples reach the depth
+    limit: the frame equals the oracle's iterative render bit for bit, and
+    differs from the recursive frame where a sample ends at the limit."""
+    s, tris, inst = level4
+    w, h, spp, depth = 96, 54, 8, 4
+    L, lp = _launch(rtx, s, w, h, spp, depth)
+    rec = L.raw_rgb().copy()
+    s.semantics(s.ITERATIVE)
+    try:
+        L.ignite()
+        raw, rpp, nrm, alb = L.raw_rgb(), L.rays_per_pixel(), L.normals(), L.albedos()
+    finally:
+        s.semantics(s.RECURSIVE)
+    sc = orc.Scene(tris, inst, s.things_spec, iterative=True)
+    ref = sc.render(orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0), w, h, spp, depth, nthreads=8)
+    exact = (raw.view(np.uint32) == ref["raw"].view(np.uint32)).all(axis=2)
+    say("iterative: bit-identical", exact.mean(), "pixels differing from recursive", (raw != rec).any(axis=2).mean())
+    assert exact.mean() >= 0.999 and (rpp == ref["rpp"]).mean() >= 0.999
+    assert (nrm.view(np.uint32) == ref["normals"].view(np.uint32)).all(axis=2).mean() >= 0.999
+    assert (alb.view(np.uint32) == ref["albedos"].view(np.uint32)).all(axis=2).mean() >= 0.999
+    assert (raw != rec).any(axis=2).mean() > 0.05
+    L.free()
+
+
 def test_render_depth_one_and_limits(rtx, level4):
     s, _, _ = level4
     L, lp = _launch(rtx, s, 64, 36, 4, 1)
