@@ -257,6 +257,21 @@ rtx_status rtx_get_semantics(rtx_ctx* ctx, int* mode);
  * of a context allocates the order's arrays (8 bytes per tile). */
 rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
 
+/* rtx_launch without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same pixels as rtx_launch; every scratch buffer of the
+ * launch (work counter, probe counts and tile order, traversal-stack
+ * overflow) lives in one static device array of 48 MiB per GPU, which the
+ * library holds until the process ends.  Limits, each rejected with
+ * RTX_ERROR_LIMIT before anything is launched: a frame rendered in probe
+ * order (as rtx_launch describes) may have at most 2^18 tiles (2^24
+ * pixels), and the scene's stack overflow (entries beyond the context's
+ * stack_lds, times the persistent grid's lanes, 4 bytes each) must fit the
+ * 46 MiB stack part.  All calls on one GPU share that GPU's array, which
+ * keeps the last call's contents (the launch clears what must start at zero
+ * with its own kernels): calls that overlap on one GPU race. */
+rtx_status rtx_launch_noalloc(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
 
 /* Batched render (no reference counterpart; SURVEY config 4): n frames, each
  * with its own launch parameters (scene handle, camera, size, spp, depth,

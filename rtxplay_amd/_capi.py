@@ -123,6 +123,7 @@ SIGNATURES = {
     "rtx_tuning_get": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_tuning_set": (c_int, [c_void_p, POINTER(TuningC)]),
     "rtx_launch": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),
+    "rtx_launch_noalloc": (c_int, [c_void_p, POINTER(LpGeneralC), c_int, c_void_p]),
     "rtx_scene_set_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_scene_get_materials": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "rtx_launch_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -114,6 +114,39 @@ struct Tmp {
 
 }  // namespace
 
+// ---- scratch of rtx_launch_noalloc ------------------------------------------
+// One static device array holds every scratch buffer of a no-allocation
+// launch: the work counter (first 256 bytes), the probe's rays per tile and
+// the tile order (kNoAllocTiles entries each), then the traversal stack's
+// overflow area (the rest).  Each GPU the library runs on holds its own copy
+// until the process ends.  Its device address is looked up once per device
+// (noalloc_setup, from rtx_ctx_create with that device current); host code
+// uses that address, never the array's name.
+constexpr size_t kNoAllocBytes = (size_t)48 << 20;
+constexpr uint32_t kNoAllocTiles = 1u << 18;   // frames of up to 2^24 pixels render in probe order
+constexpr size_t kNoAllocStackOffset = 256 + 2 * sizeof(uint32_t) * (size_t)kNoAllocTiles;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kNoAllocMaxDevices] = {};
+uint8_t* g_scratch_ptr[kNoAllocMaxDevices] = {};
+hipError_t g_scratch_err[kNoAllocMaxDevices] = {};
+
+// with `device` current: its copy's address, once per device
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? (uint8_t*)p : nullptr;
+    g_scratch_done[device] = true;
+}
+}  // namespace
+
 struct rtx_ctx {
     int device = 0;
     int num_cus = 0;
@@ -186,6 +219,7 @@ rtx_status rtx_ctx_create(int device, rtx_ctx** out) {
     HIP_OK(hipSetDevice(device));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
+    noalloc_setup(device);
     rtx_ctx* c = new rtx_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
@@ -1054,6 +1088,63 @@ rtx_status rtx_launch(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stre
         e = rtx::launch_render(b->hdr, *lp, amb, c->render_ls, (hipStream_t)stream);
     }
     if (e != hipSuccess) return hip_fail(e, "rtx_launch");
+    return RTX_SUCCESS;
+}
+
+rtx_status rtx_launch_noalloc(rtx_ctx* c, const rtx_lp_general* lp, int once, void* stream) {
+    if (!c || !lp) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: NULL argument");
+    Built* b = lookup(lp->is_handle);
+    if (!b) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch_noalloc: unknown is_handle");
+    if (b->device != c->device) return fail(RTX_ERROR_INVALID_HANDLE, "rtx_launch_noalloc: handle is on another device");
+    if (lp->image_w == 0 || lp->image_h == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: empty image");
+    if ((uint64_t)lp->image_w * lp->image_h > 0xFFFFFFFFull / 2)
+        return fail(RTX_ERROR_LIMIT, "rtx_launch_noalloc: image too large");
+    uint8_t* base = nullptr;
+    if (c->device < kNoAllocMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (g_scratch_done[c->device] && !g_scratch_ptr[c->device])
+            return hip_fail(g_scratch_err[c->device], "rtx_launch_noalloc: hipGetSymbolAddress(scratch)");
+        base = g_scratch_ptr[c->device];
+    }
+    if (!base) return fail(RTX_ERROR_LIMIT, "rtx_launch_noalloc: device " + std::to_string(c->device) +
+                                                " has no static scratch (devices 0.." +
+                                                std::to_string(kNoAllocMaxDevices - 1) + " have)");
+    // the launch's scratch inside the static array; the stack overflow area
+    // covers the entries beyond the LDS part (the picker's LDS part is never smaller)
+    rtx::LaunchScratch ls = c->render_ls;
+    const uint32_t over = b->stack_need > ls.stack_lds ? b->stack_need - ls.stack_lds : 1u;
+    const size_t stack_bytes = sizeof(uint32_t) * (size_t)over * ls.gstride;
+    if (stack_bytes > kNoAllocBytes - kNoAllocStackOffset)
+        return fail(RTX_ERROR_LIMIT, "rtx_launch_noalloc: the scene's traversal stack needs " +
+                                         std::to_string(stack_bytes) + " bytes of overflow for " +
+                                         std::to_string(ls.gstride) + " lanes, more than the " +
+                                         std::to_string(kNoAllocBytes - kNoAllocStackOffset) + " of the static scratch");
+    ls.counter = (uint32_t*)base;
+    ls.tile_cost = (uint32_t*)(base + 256);
+    ls.tile_order = ls.tile_cost + kNoAllocTiles;
+    ls.tile_cap = kNoAllocTiles;
+    ls.gstack = (uint32_t*)(base + kNoAllocStackOffset);
+    ls.gstack_entries = over;
+    HIP_OK(hipSetDevice(c->device));
+    const float3 amb = make_float3(c->ambient.x, c->ambient.y, c->ambient.z);
+    hipError_t e;
+    if (once || lp->picker) {
+        if (!lp->pick_id) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: picker without pick_id");
+        if (lp->pick_x >= lp->image_w || lp->pick_y >= lp->image_h)
+            return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: pick position outside the image");
+        e = rtx::launch_pick(b->hdr, *lp, ls, (hipStream_t)stream);
+    } else {
+        if (lp->spp == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: spp == 0");
+        if (lp->depth == 0) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: depth == 0");
+        if (!lp->rawRGB) return fail(RTX_ERROR_INVALID_VALUE, "rtx_launch_noalloc: rawRGB is NULL");
+        if (rtx::frame_ordered(*lp, ls) && rtx::frame_tiles(*lp, ls.tile_w) > kNoAllocTiles)
+            return fail(RTX_ERROR_LIMIT, "rtx_launch_noalloc: a frame rendered in probe order has " +
+                                             std::to_string(rtx::frame_tiles(*lp, ls.tile_w)) +
+                                             " tiles, more than the " + std::to_string(kNoAllocTiles) +
+                                             " of the static scratch");
+        e = rtx::launch_render(b->hdr, *lp, amb, ls, (hipStream_t)stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "rtx_launch_noalloc");
     return RTX_SUCCESS;
 }
 
