@@ -7,18 +7,35 @@ acceleration structure already built and resident in HBM.
 
 Prints ONE JSON line on stdout (progress goes to stderr).  value = traced rays
 per second (hits + misses, counted as the reference's rays-per-pixel AOV) in
-Mrays/s; samples/s beside it.  roofline: algorithmic bytes per ray (4-wide
-node visits x 112 B + triangle tests x 48 B + shading fetch 80 B + ray in/out
-48 B, visits and tests counted by a statistics launch) x rays/s against the
-8 TB/s HBM peak.  cpu_baseline: the
-oracle's renderer (C, same scene, same semantics) on the host cores over a
-bounded sample of rows.
+Mrays/s; samples/s beside it.
+
+roofline: the render kernel is bound by VALU issue (counters: most issue
+slots busy, half the lanes idle, HBM at a third of its peak), so `bound` is
+"valu": achieved = VALU wave-instructions per ray (from the committed counter
+profile of the same kernel build, stamped with a hash of the kernel sources)
+x rays per second measured here, against 256 CUs x 2.4 GHz x one wave
+instruction per CU per clock.  Beside it: the share of lanes active per VALU
+instruction, the HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE
+counters (scaled by a copy kernel of known size) against the 8 TB/s peak,
+and the bytes per ray that the traversal touches (node visits x 112 B +
+triangle tests x 48 B + shading 80 B + ray in/out 48 B: bytes served by
+L1, L2 and the MALL, most of them never from HBM).
+
+cpu_baseline: the oracle's renderer (C, same scene, same semantics) on the
+host cores over a bounded sample of rows; cpu_reference: the reference's
+own rtow.cxx built from its sources; cpu_c1: the rtow.cxx tracer at config
+1 (400x225, 10 spp, 16 rays per sample) in FP64 as written and in FP32 on
+one and on all threads.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c3|c2]
+    python bench.py --counters      # the counter profile (rocprofv3 passes)
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -34,38 +51,91 @@ WORKLOADS = {
     "c2": (4, 1280, 720, 50, 16),
 }
 HBM_PEAK_GBS = 8000.0
+# VALU issue: one wave instruction per CU per clock, 256 CUs at the 2.4 GHz peak clock
+CUS, PEAK_GHZ = 256, 2.4
+VALU_PEAK_GINST = CUS * PEAK_GHZ
 # bytes per 4-wide node visit (6 plane float4 + the refs float4), per
 # triangle test, per shading step (hit triangle 48 + instance record 32) and
 # per ray for its own input/output (ray 32 in, hit 16 out)
 NODE_BYTES, TRI_BYTES, SHADE_BYTES, IO_BYTES = 112, 48, 80, 48
+PROFILES = os.path.join(ROOT, "profiles")
+FRAME_KERNEL = "k_render<256, false, 32, false>"
+COPY_KERNEL = "k_pp<false>"
+# counter passes of the headline frame (each within one pass's block limits)
+PASSES = [
+    ("sq", ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY",
+            "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_ANY", "SQ_CYCLES"]),
+    ("cache", ["TCC_HIT_sum", "TCC_MISS_sum", "TCP_TCC_READ_REQ_sum", "TCP_TOTAL_CACHE_ACCESSES_sum"]),
+    ("fetch", ["FETCH_SIZE"]),
+    ("write", ["WRITE_SIZE"]),
+]
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(level, w, h, spp, depth, seconds, threads):
+def kernel_stamp():
+    """Hash of the sources and build flags the kernels are compiled from: a
+    counter profile holds for the build whose stamp it carries."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "rtxplay_amd", "csrc", "*"))) + [
+        os.path.join(ROOT, "rtxplay_amd", "build.py")]
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+# ---------------------------------------------------------------------------
+# CPU legs
+# ---------------------------------------------------------------------------
+class OracleFrame:
+    """The oracle's scene and camera of a workload (built once, used by the
+    CPU baseline and the RMSE reference)."""
+
+    def __init__(self, level, w, h):
+        import oracle
+        from rtxplay_amd.scene import sphere_things
+        t0 = time.perf_counter()
+        things = sphere_things(0, 11)
+        meshes = {level: oracle.sphere_mesh(1.0, level)}
+        tris, inst = oracle.flatten(meshes, things, [level] * len(things))
+        self.scene = oracle.Scene(tris, inst, things)
+        self.cam = oracle.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0)
+        self.w, self.h, self.level, self.ntris = w, h, level, len(tris)
+        log("oracle scene: %d triangles, BVH in %.1f s" % (len(tris), time.perf_counter() - t0))
+
+
+def cpu_baseline(of, spp, depth, seconds, threads):
     """The oracle renderer on host cores over a bounded sample of rows."""
-    import oracle
-    from rtxplay_amd.scene import sphere_things
-    things = sphere_things(0, 11)
-    meshes = {level: oracle.sphere_mesh(1.0, level)}
-    tris, inst = oracle.flatten(meshes, things, [level] * len(things))
-    log("cpu baseline: oracle BVH over %d triangles" % len(tris))
-    sc = oracle.Scene(tris, inst, things)
-    cam = oracle.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0)
+    w, h = of.w, of.h
     order = np.random.default_rng(0).permutation(h // threads)
     rays, rows, t0 = 0, 0, time.perf_counter()
     for band in order:
         y0 = int(band) * threads
-        rays += sc.render(cam, w, h, spp, depth, y0=y0, y1=y0 + threads, nthreads=threads)["rays"]
+        rays += of.scene.render(of.cam, w, h, spp, depth, y0=y0, y1=y0 + threads, nthreads=threads)["rays"]
         rows += threads
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%d random rows of %dx%d at %d spp depth %d on the same level-%d scene (%.1f s, %d rays)"
-                      % (rows, w, h, spp, depth, level, dt, rays)}
+    rec = {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": "%d random rows of %dx%d at %d spp depth %d on the same level-%d scene (%.1f s, %d rays)"
+                     % (rows, w, h, spp, depth, of.level, dt, rays)}
+    rec.update(host_info())
+    return rec
 
 
 def cpu_reference(seconds):
@@ -77,7 +147,6 @@ def cpu_reference(seconds):
     reproduces the tracer's image byte for byte.  None when the binary is
     not built."""
     import select
-    import subprocess
     exe = os.path.join(ROOT, "oracle", "_ref", "rtow")
     if not os.path.exists(exe):
         return None
@@ -111,9 +180,218 @@ def cpu_reference(seconds):
         return None
     dt = starts[done]
     rays = oracle.rtow_rays(done * w, w, h, 10, 50)
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "reference",
-            "sample": "rtow.cxx built from the reference sources (g++ -O2): its first %d rows of 1280x720 at "
-                      "10 spp, depth 50, FP64 analytic spheres (%.1f s, %d rays)" % (done, dt, rays)}
+    rec = {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "reference",
+           "sample": "rtow.cxx built from the reference sources (g++ -O2): its first %d rows of 1280x720 at "
+                     "10 spp, depth 50, FP64 analytic spheres (%.1f s, %d rays)" % (done, dt, rays)}
+    rec.update(host_info())
+    return rec
+
+
+def cpu_c1(threads):
+    """Config 1: rtow.cxx's tracer (analytic spheres, its own scene and
+    camera) at 400x225, 10 spp, 16 rays per sample (trace depth 15), in
+    three forms: FP64 on the reference's glibc stream, one thread (rtow.cxx
+    as written); FP32 on per-pixel streams, one thread; FP32 on all host
+    threads."""
+    import oracle
+    w, h, spp, depth = 400, 225, 10, 15
+    rec = {"workload": "rtow.cxx scene, analytic spheres, %dx%d, %d spp, at most 16 rays per sample" % (w, h, spp)}
+    rec.update(host_info())
+    for name, kw in (("fp64_1thread", dict(fp32=False, nthreads=0)), ("fp32_1thread", dict(fp32=True, nthreads=1)),
+                     ("fp32_allcores", dict(fp32=True, nthreads=threads))):
+        # whole frames until at least a second has passed (the all-core frame takes ~0.1 s)
+        t0, reps, rays = time.perf_counter(), 0, 0
+        while reps == 0 or time.perf_counter() - t0 < 1.0:
+            r = oracle.rtow_render_ex(w, h, spp, depth, **kw)
+            reps += 1
+            rays += r["rays"]
+        dt = time.perf_counter() - t0
+        rec[name] = {"threads": max(1, kw["nthreads"]), "frames": reps, "seconds": round(dt, 3),
+                     "Mrays_per_s": round(rays / dt / 1e6, 3),
+                     "Msamples_per_s": round(reps * w * h * spp / dt / 1e6, 3),
+                     "rays_per_sample": round(r["rays"] / (w * h * spp), 4),
+                     "image_mean": [round(float(x), 5) for x in r["mean"].reshape(-1, 3).mean(axis=0)]}
+        log("c1 %s: %.2f Mrays/s (%.2f s)" % (name, rec[name]["Mrays_per_s"], dt))
+    return rec
+
+
+def rmse_leg(of, gpu_raw, spp, depth, threads, rows=2, ref_spp=4096):
+    """RMSE of the GPU frame against a 4096-spp CPU reference of the same
+    scene on `rows` rows spread over the image (the oracle with per-pixel
+    streams independent of the frame's), beside the RMSE of an independent
+    CPU render at the frame's spp and the RMSE that the Monte Carlo
+    variance alone predicts, sqrt(mean var * (1/spp + 1/4096))."""
+    w, h = of.w, of.h
+    ys = [int((k + .5) * h / rows) for k in range(rows)]
+    t0 = time.perf_counter()
+    ref, cpu, var = [], [], []
+    for y in ys:
+        r = of.scene.render(of.cam, w, h, ref_spp, depth, y0=y, y1=y + 1, nthreads=threads, want_sumsq=True,
+                            seed_base=1 << 24)
+        ref.append(r["raw"][y].astype(np.float64))
+        var.append(np.maximum(r["sumsq"][y].astype(np.float64) / ref_spp - ref[-1] ** 2, 0))
+        c = of.scene.render(of.cam, w, h, spp, depth, y0=y, y1=y + 1, nthreads=threads, seed_base=2 << 24)
+        cpu.append(c["raw"][y].astype(np.float64))
+    ref, cpu, var = np.array(ref), np.array(cpu), np.array(var)
+    g = gpu_raw[ys].astype(np.float64)
+    rm_g = float(np.sqrt(((g - ref) ** 2).mean()))
+    rm_c = float(np.sqrt(((cpu - ref) ** 2).mean()))
+    rm_e = float(np.sqrt(var.mean() * (1.0 / spp + 1.0 / ref_spp)))
+    log("rmse vs %d spp: gpu %.5f, cpu %.5f, variance predicts %.5f (%.1f s)" % (ref_spp, rm_g, rm_c, rm_e,
+                                                                                 time.perf_counter() - t0))
+    return {"reference": "oracle render at %d spp with independent streams, rows %s of %dx%d" % (ref_spp, ys, w, h),
+            "rmse_gpu": round(rm_g, 6), "rmse_cpu_same_spp": round(rm_c, 6), "rmse_predicted": round(rm_e, 6),
+            "gpu_over_predicted": round(rm_g / rm_e, 4), "spp": spp,
+            "hit_t_tolerance": "closest hits are bit-exact on decidable rays; hit t within 1e-5 relative on 99.97 % "
+                               "of hits, the rest within 2^-15 of the coordinate magnitude along the ray (FP32 "
+                               "coordinates near a 13-unit camera; worst 2.3e-5 relative)"}
+
+
+# ---------------------------------------------------------------------------
+# counter profile (rocprofv3 passes over the headline frame)
+# ---------------------------------------------------------------------------
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _counter_means(outdir, kernel):
+    """Mean per dispatch of each counter over the dispatches of `kernel`."""
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no counter_collection.csv under " + outdir)
+    per = {}
+    for f in files:
+        for r in _csv_rows(f):
+            if kernel not in r.get("Kernel_Name", ""):
+                continue
+            d = per.setdefault(r["Counter_Name"], {})
+            key = (f, r.get("Dispatch_Id", ""))
+            d[key] = d.get(key, 0.0) + float(r["Counter_Value"])
+    return {k: sum(v.values()) / len(v) for k, v in per.items()}, {k: len(v) for k, v in per.items()}
+
+
+def _kernel_stats(outdir):
+    files = glob.glob(os.path.join(outdir, "**", "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no kernel_stats.csv under " + outdir)
+    return _csv_rows(files[0])
+
+
+def counters_mode(args):
+    """Runs each counter pass and the kernel-trace pass of the headline
+    frame under rocprofv3 in its own process (under a kill timeout), plus
+    the copy probe's FETCH_SIZE and WRITE_SIZE passes, and writes
+    <out>/k_render_<workload>_counters.json and .txt (copied to profiles/
+    for the committed record)."""
+    out = os.path.abspath(args.counters_out)
+    os.makedirs(out, exist_ok=True)
+    bench = os.path.abspath(__file__)
+    frame = [sys.executable, bench, "--legs", "frame", "--steps", "1", "--warmup", "0", "--no-cpu", "--stats-spp", "1",
+             "--workload", args.workload, "--no-rmse"]
+    copy = [sys.executable, bench, "--copy-probe"]
+    cmds, res, rays = [], {}, None
+
+    def run(name, pre, cmd):
+        d = os.path.join(out, "raw_" + name)
+        full = ["timeout", "-s", "KILL", "300", "rocprofv3"] + pre + ["-f", "csv", "-d", d, "-o", "run", "--"] + cmd
+        shown = " ".join(full[4:]).replace(sys.executable, "python3").replace(bench, "bench.py").replace(out, "OUT")
+        cmds.append(shown)
+        log("counters: " + shown)
+        p = subprocess.run(full, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("rocprofv3 pass %s failed (%d): %s" % (name, p.returncode, p.stderr[-2000:]))
+        return d, p.stdout
+
+    for name, counters in PASSES:
+        d, so = run(name, ["--pmc"] + counters, frame)
+        for line in so.splitlines():
+            if line.startswith("{"):
+                rays = json.loads(line).get("rays_per_frame")
+        m, n = _counter_means(d, FRAME_KERNEL)
+        res.update(m)
+    cf = {}
+    for name, ctr in (("copy_fetch", "FETCH_SIZE"), ("copy_write", "WRITE_SIZE")):
+        d, _ = run(name, ["--pmc", ctr], copy)
+        m, n = _counter_means(d, COPY_KERNEL)
+        cf[ctr] = m[ctr] * n[ctr]                 # KiB over all the probe's launches
+    d, _ = run("trace", ["--kernel-trace", "--stats"], frame[:5] + ["2"] + frame[6:])
+    stats = _kernel_stats(d)
+    npx = 64 << 20
+    read_factor = 3 * 12 * npx / (cf["FETCH_SIZE"] * 1024.0)     # known bytes / counted bytes
+    write_factor = 3 * 4 * npx / (cf["WRITE_SIZE"] * 1024.0)
+    hbm = res["FETCH_SIZE"] * 1024.0 * read_factor + res["WRITE_SIZE"] * 1024.0 * write_factor
+    fr = [r for r in stats if FRAME_KERNEL in r.get("Name", r.get("KERNEL_NAME", ""))]
+    avg_us = float(fr[0].get("AverageNs", fr[0].get("AVERAGE_NS", 0))) / 1e3 if fr else None
+    summary = {"workload": args.workload, "kernel": FRAME_KERNEL, "stamp": kernel_stamp(), "rays_per_launch": rays,
+               "counters_per_launch": res, "copy_probe_kib": cf, "read_factor": read_factor,
+               "write_factor": write_factor, "hbm_bytes_per_launch": hbm, "kernel_avg_us": avg_us,
+               "commands": cmds}
+    jpath = os.path.join(out, "k_render_%s_counters.json" % args.workload)
+    json.dump(summary, open(jpath, "w"), indent=1, sort_keys=True)
+    lines = ["# counters of the headline frame kernel %s, workload %s (kernel sources stamp %s)"
+             % (FRAME_KERNEL, args.workload, summary["stamp"]), "# commands (each its own process; OUT = the output "
+             "directory of: python3 bench.py --counters --counters-out OUT):"]
+    lines += ["#   " + c for c in cmds]
+    lines.append("# rays per launch %s; values are means per dispatch of %s" % (rays, FRAME_KERNEL))
+    lines.append("%-32s %16s %12s" % ("counter", "per launch", "per ray"))
+    for k in sorted(res):
+        lines.append("%-32s %16.4e %12.3f" % (k, res[k], res[k] / rays if rays else float("nan")))
+    lines.append("copy probe (k_pp<false>, 64 Mi pixels, 768 MiB read + 256 MiB written per launch, 3 launches): "
+                 "FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB -> read factor %.4f, write factor %.4f"
+                 % (cf["FETCH_SIZE"], cf["WRITE_SIZE"], read_factor, write_factor))
+    lines.append("HBM bytes per launch %.4e (FETCH_SIZE x 1024 x read factor + WRITE_SIZE x 1024 x write factor)" % hbm)
+    s = res
+    if rays:
+        lines.append("VALU wave instructions per ray %.2f; issue fraction at %d CUs x %.1f GHz: see the bench line"
+                     % (s["SQ_INSTS_VALU"] / rays, CUS, PEAK_GHZ))
+    lines.append("lanes active per VALU instruction: SQ_THREAD_CYCLES_VALU / SQ_INSTS_VALU / 64 = %.3f"
+                 % (s["SQ_THREAD_CYCLES_VALU"] / s["SQ_INSTS_VALU"] / 64))
+    lines.append("waves waiting: SQ_WAIT_ANY / SQ_WAVE_CYCLES = %.3f; issuing: SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES = %.3f"
+                 % (s["SQ_WAIT_ANY"] / s["SQ_WAVE_CYCLES"], s["SQ_ACTIVE_INST_ANY"] / s["SQ_WAVE_CYCLES"]))
+    lines.append("L2 hit rate %.3f; L1 requests going to L2 %.3f"
+                 % (s["TCC_HIT_sum"] / (s["TCC_HIT_sum"] + s["TCC_MISS_sum"]),
+                    s["TCP_TCC_READ_REQ_sum"] / s["TCP_TOTAL_CACHE_ACCESSES_sum"]))
+    lines.append("kernel trace: %s average %.1f us over the timed frames" % (FRAME_KERNEL, avg_us or float("nan")))
+    lines.append("")
+    lines.append("kernel trace summary (rocprofv3 --kernel-trace --stats, 2 frames):")
+    keys = list(stats[0].keys()) if stats else []
+    for r in stats[:12]:
+        lines.append("  " + "  ".join("%s=%s" % (k, r[k]) for k in keys))
+    open(os.path.join(out, "k_render_%s_counters.txt" % args.workload), "w").write("\n".join(lines) + "\n")
+    print(json.dumps({"counters": jpath}))
+
+
+def roofline(workload, rays, ms, b_ray):
+    """The roofline record of the frame kernel from the committed counter
+    profile (None fields when it is missing or stamped for other sources)."""
+    path = os.path.join(PROFILES, "k_render_%s_counters.json" % workload)
+    cache = {"bytes_per_ray": round(b_ray, 1), "GB_per_s": round(b_ray * rays / (ms * 1e-3) / 1e9, 1),
+             "note": "bytes the traversal and shading touch per ray, served by L1, L2 and the MALL; "
+                     "most never come from HBM"}
+    rec = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINST, "unit": "G VALU wave-instructions/s",
+           "frac": None, "traffic": None, "lanes_active": None, "source": os.path.relpath(path, ROOT),
+           "stamp": kernel_stamp(), "stale": True, "cache_level": cache, "hbm": None}
+    if not os.path.exists(path):
+        return rec
+    p = json.load(open(path))
+    if p.get("stamp") != rec["stamp"] or not p.get("rays_per_launch"):
+        return rec
+    c = p["counters_per_launch"]
+    per_ray = c["SQ_INSTS_VALU"] / p["rays_per_launch"]
+    ach = per_ray * rays / (ms * 1e-3) / 1e9
+    traffic = p["hbm_bytes_per_launch"] * rays / p["rays_per_launch"]
+    rec.update({"achieved": round(ach, 2), "frac": round(ach / VALU_PEAK_GINST, 4), "traffic": traffic,
+                "valu_inst_per_ray": round(per_ray, 2),
+                "lanes_active": round(c["SQ_THREAD_CYCLES_VALU"] / c["SQ_INSTS_VALU"] / 64, 4), "stale": False,
+                "waves_waiting": round(c["SQ_WAIT_ANY"] / c["SQ_WAVE_CYCLES"], 4),
+                "hbm": {"achieved": round(traffic / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "read_factor": round(p["read_factor"], 4), "write_factor": round(p["write_factor"], 4),
+                        "l2_hit": round(c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), 4)},
+                "profile_kernel_avg_ms": round(p["kernel_avg_us"] / 1e3, 3) if p.get("kernel_avg_us") else None})
+    return rec
 
 
 TUNED_JSON = os.path.join(ROOT, "rtxplay_amd", "tuned.json")
@@ -122,18 +400,17 @@ ARCH = "gfx950"
 
 def batched_leg(reps, retune, log_tune):
     """The 64-frame batch: 64 scenes (seeds 0..63) and cameras of uneven size
-    rendered by one hipGraph-captured persistent launch with autotuned launch
-    parameters, against the same 64 frames as 64 plain launches back to back
-    on one stream (the plain path's default parameters).  Returns the record."""
+    rendered by one hipGraph-captured persistent launch, against the same 64
+    frames as 64 plain launches back to back on one stream (the plain path's
+    default parameters).  The batch runs with the library's default tuning
+    and with the autotuner's tuning, which is chosen on 64 other frames
+    (seeds 64..127, the same generator): the frames it is scored on play no
+    part in choosing it.  Returns the record."""
     import rtxplay_amd as rtx
     from rtxplay_amd import autotune
     from rtxplay_amd.hip import Event, Stream
     from rtxplay_amd.scene import BatchFrames, batch_frame
 
-    t0 = time.perf_counter()
-    bf = BatchFrames([batch_frame(i) for i in range(64)])
-    log("batch: 64 scenes, %d triangles, %d samples, built in %.1f s"
-        % (bf.triangles(), bf.samples(), time.perf_counter() - t0))
     stream = Stream()
 
     def timed(fn):
@@ -147,6 +424,24 @@ def batched_leg(reps, retune, log_tune):
         e1.synchronize()
         return e0.elapsed_ms(e1) / reps
 
+    tuned = None if retune else autotune.load(TUNED_JSON, ARCH, "batch64-heldout")
+    source = "stored (tuned on the held-out frames, seeds 64..127)"
+    if tuned is None:
+        t0 = time.perf_counter()
+        held = BatchFrames([batch_frame(i) for i in range(64, 128)])
+        log("batch: held-out frames (seeds 64..127) built in %.1f s" % (time.perf_counter() - t0))
+        tuned, tsec, hist = autotune.tune_batch(held.scenes[0], held.lps, log=log_tune)
+        autotune.save(TUNED_JSON, ARCH, "batch64-heldout", tuned, tsec,
+                      note="coordinate descent over %d configurations on the held-out frames (seeds 64..127)"
+                           % len(hist))
+        held.close()
+        source = "autotuned now on the held-out frames, seeds 64..127 (%d configurations)" % len(hist)
+
+    t0 = time.perf_counter()
+    bf = BatchFrames([batch_frame(i) for i in range(64)])
+    log("batch: 64 scenes, %d triangles, %d samples, built in %.1f s"
+        % (bf.triangles(), bf.samples(), time.perf_counter() - t0))
+
     def sequential():
         for l in bf.launchers:
             l.ignite(stream=stream, sync=False)
@@ -156,31 +451,21 @@ def batched_leg(reps, retune, log_tune):
     log("batch: 64 plain launches %.1f ms (%d rays)" % (seq_ms, rays))
 
     ctx = bf.scenes[0]
-    default = {k: ctx.tuning()[k] for k in autotune.KEYS}
     b = rtx.Batch(ctx, bf.lps, use_graph=True)
     def_ms = timed(lambda: b.launch(stream, sync=False))
     b.free()
     log("batch: one launch, default tuning %.1f ms" % def_ms)
-
-    tuned = None if retune else autotune.load(TUNED_JSON, ARCH, "batch64")
-    source = "stored"
-    if tuned is None:
-        # on the 64 frames themselves: a quarter-spp proxy ranked the shading
-        # batch sizes differently from the full frames
-        tuned, tsec, hist = autotune.tune_batch(ctx, bf.lps, log=log_tune)
-        autotune.save(TUNED_JSON, ARCH, "batch64", tuned, tsec,
-                      note="coordinate descent over %d configurations on the 64 frames" % len(hist))
-        source = "autotuned now (%d configurations)" % len(hist)
     ctx.tuning(**tuned)
     b = rtx.Batch(ctx, bf.lps, use_graph=True)
     bat_ms = timed(lambda: b.launch(stream, sync=False))
     b.free()
-    log("batch: one launch, tuned %s: %.1f ms" % (tuned, bat_ms))
+    log("batch: one launch, tuning %s (%s): %.1f ms" % (tuned, source, bat_ms))
     # the batch reproduces the plain launches: rays per pixel summed over all frames
     rays_b = bf.rays()
     rec = {"frames": 64, "triangles": bf.triangles(), "samples": bf.samples(), "rays": rays,
            "sequential_ms": round(seq_ms, 2), "batched_default_ms": round(def_ms, 2), "batched_ms": round(bat_ms, 2),
-           "speedup": round(seq_ms / bat_ms, 3), "Mrays_per_s_batched": round(rays / (bat_ms * 1e-3) / 1e6, 1),
+           "speedup": round(seq_ms / bat_ms, 3), "speedup_default_tuning": round(seq_ms / def_ms, 3),
+           "Mrays_per_s_batched": round(rays / (bat_ms * 1e-3) / 1e6, 1),
            "Mrays_per_s_sequential": round(rays / (seq_ms * 1e-3) / 1e6, 1), "tuning": tuned,
            "tuning_source": source, "same_rays": rays_b == rays}
     bf.close()
@@ -364,92 +649,10 @@ def two_level_leg(spp=64, steps=3):
     return rec
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stats-spp", type=int, default=16)
-    ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
-    ap.add_argument("--legs", default="frame,batch,diff,interactive,two_level",
-                    help="comma list of: frame (the headline render), batch (64 frames), diff (adjoint), "
-                         "interactive (repeated 1-spp launches), two_level (flat vs two-level layouts)")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
-    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
-    ap.add_argument("--batch-reps", type=int, default=3)
-    ap.add_argument("--retune", action="store_true", help="re-run the batch autotuner (else use the stored tuning)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="run only the post-process kernel over 64M pixels (768 MiB read, 256 MiB written): "
-                         "a plain streaming kernel of known size that calibrates the HBM byte counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (pixels would be sharded "
-              "by row bands with the BVH replicated per GPU); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    import rtxplay_amd as rtx
-
-    level, w, h, spp, depth = WORKLOADS[args.workload]
-    if rtx.device_count() < 1:
-        sys.exit("bench.py needs a GPU")
-    if args.copy_probe:
-        n = 64 << 20
-        src, dst = rtx.DeviceBuffer(12 * n), rtx.DeviceBuffer(4 * n)
-        src.zero()
-        for _ in range(3):
-            rtx.pp_none(src.ptr, dst.ptr, 8192, n // 8192)
-        rtx.synchronize()
-        print(json.dumps({"copy_probe": {"pixels": n, "read_bytes": 12 * n, "write_bytes": 4 * n}}))
-        return
-    legs = set(x for x in args.legs.split(",") if x)
-    if args.no_batch:
-        legs.discard("batch")
-    if args.no_diff:
-        legs.discard("diff")
-    fr = frame_leg(args, level, w, h, spp, depth) if "frame" in legs else None
-    batched = None
-    if "batch" in legs:
-        batched = batched_leg(args.batch_reps, args.retune, log)
-        log("batched speedup %.2fx" % batched["speedup"])
-    diff = diff_leg(max(3, args.batch_reps)) if "diff" in legs else None
-    inter = interactive_leg() if "interactive" in legs else None
-    two = two_level_leg() if "two_level" in legs else None
-
-    cpu = cpu_ref = None
-    if not args.no_cpu and fr is not None:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(level, w, h, spp, depth, args.cpu_seconds, threads)
-        log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
-        cpu_ref = cpu_reference(args.cpu_seconds)
-        if cpu_ref:
-            log("reference cpu tracer: %.2f Mrays/s on 1 thread" % cpu_ref["value"])
-
-    fr = fr or {}
-    out = {
-        "metric": METRIC, "value": fr.get("mrays"), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": fr.get("ms"), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "rtow-final-scene level-%d spheres (%s triangles), %dx%d, %d spp, depth %d, seed 0"
-                               % (level, fr.get("triangles"), w, h, spp, depth),
-                   "image": [w, h], "spp": spp, "depth": depth, "triangles": fr.get("triangles"),
-                   "parallelism": "single-gpu"},
-        "samples_per_s": fr.get("samples_per_s"), "Msamples_per_s": fr.get("Msamples_per_s"),
-        "rays_per_frame": fr.get("rays"), "rays_per_sample": fr.get("rays_per_sample"),
-        "bvh": fr.get("bvh"), "roofline": fr.get("roofline"),
-        "cpu_baseline": cpu, "cpu_reference": cpu_ref,
-        "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
-        "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
-        "interactive": inter, "two_level": two,
-    }
-    print(json.dumps(out), flush=True)
-
-
 def frame_leg(args, level, w, h, spp, depth):
     """The headline frame: build, warmup, K timed renders, then a statistics
-    launch at reduced spp for the per-ray node/triangle counts."""
+    launch at reduced spp for the per-ray node/triangle counts.  Returns the
+    record and the last frame's rawRGB."""
     import rtxplay_amd as rtx
     from rtxplay_amd.hip import Event, Stream
     from rtxplay_amd.scene import RTOWScene
@@ -485,12 +688,13 @@ def frame_leg(args, level, w, h, spp, depth):
     wall = time.perf_counter() - wall0
     ms = e0.elapsed_ms(e1) / max(args.steps, 1)
     rpp = launcher.rays_per_pixel()
+    raw = launcher.raw_rgb().copy()
     rays = int(rpp.sum(dtype=np.uint64))
     log("timed %d steps: %.1f ms/step (wall %.2f s), %d rays/frame" % (args.steps, ms, wall, rays))
     mrays = rays / (ms * 1e-3) / 1e6
     samples = w * h * spp / (ms * 1e-3)
 
-    # bytes per ray from a statistics launch at reduced spp (per-ray averages)
+    # node visits and triangle tests per ray from a statistics launch at reduced spp
     lp.spp = args.stats_spp
     nodes, tris = launcher.ignite_counted(stream=stream)
     srays = int(launcher.rays_per_pixel().sum(dtype=np.uint64))
@@ -498,12 +702,10 @@ def frame_leg(args, level, w, h, spp, depth):
     n_node = float(nodes.sum(dtype=np.uint64)) / srays
     n_tri = float(tris.sum(dtype=np.uint64)) / srays
     b_ray = n_node * NODE_BYTES + n_tri * TRI_BYTES + SHADE_BYTES + IO_BYTES
-    achieved = b_ray * rays / (ms * 1e-3) / 1e9
-    log("per ray: %.1f node visits, %.1f triangle tests -> %.0f B/ray, %.0f GB/s" % (n_node, n_tri, b_ray, achieved))
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "render_%s_traffic.json" % args.workload)
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    rf = roofline(args.workload, rays, ms, b_ray)
+    rf.update({"node_visits_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2)})
+    log("per ray: %.1f node visits, %.1f triangle tests; roofline %s" % (n_node, n_tri, {k: rf[k] for k in (
+        "bound", "achieved", "frac", "lanes_active", "stale")}))
     launcher.free()
     scene.close()
     return {"mrays": round(mrays, 2), "ms": round(ms, 3), "triangles": st["num_prims"],
@@ -511,10 +713,142 @@ def frame_leg(args, level, w, h, spp, depth):
             "rays_per_sample": round(rays / (w * h * spp), 4),
             "bvh": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "max_depth": st["max_depth"],
                     "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1)},
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                         "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 2),
-                         "tri_tests_per_ray": round(n_tri, 2)}}
+            "roofline": rf}, raw
+
+
+def c2_leg(steps=5):
+    """Config 2: the level-4 scene (497,664 triangles), 1280x720, 50 spp,
+    depth 16: build, one warmup, `steps` timed launches."""
+    import rtxplay_amd as rtx
+    from rtxplay_amd.hip import Event, Stream
+    from rtxplay_amd.scene import RTOWScene
+    level, w, h, spp, depth = WORKLOADS["c2"]
+    sc = RTOWScene(seed=0, half=11, level=level)
+    sc.load()
+    lp = rtx.LpGeneral()
+    lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, depth, sc.build()
+    st = sc.stats()
+    lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, w / h, 0.1, 10.0).c
+    la = rtx.Launcher(sc, lp)
+    stream = Stream()
+    la.ignite(stream=stream)
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(steps):
+        la.ignite(stream=stream, sync=False)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    rays = int(la.rays_per_pixel().sum(dtype=np.uint64))
+    la.free()
+    sc.close()
+    rec = {"workload": "rtow-final-scene level-4 spheres (%d triangles), %dx%d, %d spp, depth %d, seed 0"
+                       % (st["num_prims"], w, h, spp, depth), "ms_per_frame": round(ms, 3),
+           "Mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
+           "Msamples_per_s": round(w * h * spp / (ms * 1e-3) / 1e6, 1), "rays_per_frame": rays,
+           "build_ms": round(st["build_ms"], 1)}
+    log("c2: %s" % rec)
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against the 4096-spp CPU reference")
+    ap.add_argument("--stats-spp", type=int, default=16)
+    ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
+    ap.add_argument("--legs", default="frame,c2,batch,diff,interactive,two_level",
+                    help="comma list of: frame (the headline render), c2 (config 2), batch (64 frames), "
+                         "diff (adjoint), interactive (repeated 1-spp launches), two_level (flat vs two-level "
+                         "layouts)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable (adjoint) leg")
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--retune", action="store_true",
+                    help="re-run the batch autotuner on the held-out frames (seeds 64..127) and store it")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only the post-process kernel over 64M pixels (768 MiB read, 256 MiB written): "
+                         "a plain streaming kernel of known size that calibrates the HBM byte counters")
+    ap.add_argument("--counters", action="store_true",
+                    help="run the rocprofv3 counter passes of the headline frame and write their summary")
+    ap.add_argument("--counters-out", default=os.path.join(ROOT, "gpurun_out", "counters"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (pixels would be sharded "
+              "by row bands with the BVH replicated per GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.counters:
+        counters_mode(args)
+        return
+
+    import rtxplay_amd as rtx
+
+    level, w, h, spp, depth = WORKLOADS[args.workload]
+    if rtx.device_count() < 1:
+        sys.exit("bench.py needs a GPU")
+    if args.copy_probe:
+        n = 64 << 20
+        src, dst = rtx.DeviceBuffer(12 * n), rtx.DeviceBuffer(4 * n)
+        src.zero()
+        for _ in range(3):
+            rtx.pp_none(src.ptr, dst.ptr, 8192, n // 8192)
+        rtx.synchronize()
+        print(json.dumps({"copy_probe": {"pixels": n, "read_bytes": 12 * n, "write_bytes": 4 * n}}))
+        return
+    legs = set(x for x in args.legs.split(",") if x)
+    if args.no_batch:
+        legs.discard("batch")
+    if args.no_diff:
+        legs.discard("diff")
+    fr, raw = frame_leg(args, level, w, h, spp, depth) if "frame" in legs else (None, None)
+    c2 = c2_leg() if "c2" in legs and args.workload == "c3" else None
+    batched = None
+    if "batch" in legs:
+        batched = batched_leg(args.batch_reps, args.retune, log)
+        log("batched speedup %.2fx" % batched["speedup"])
+    diff = diff_leg(max(3, args.batch_reps)) if "diff" in legs else None
+    inter = interactive_leg() if "interactive" in legs else None
+    two = two_level_leg() if "two_level" in legs else None
+
+    cpu = cpu_ref = c1 = rmse = None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if fr is not None and not (args.no_cpu and args.no_rmse):
+        of = OracleFrame(level, w, h)
+        if not args.no_rmse:
+            rmse = rmse_leg(of, raw, spp, depth, threads)
+        if not args.no_cpu:
+            cpu = cpu_baseline(of, spp, depth, args.cpu_seconds, threads)
+            log("cpu baseline: %.2f Mrays/s on %d threads" % (cpu["value"], threads))
+        del of
+    if fr is not None and not args.no_cpu:
+        cpu_ref = cpu_reference(args.cpu_seconds)
+        if cpu_ref:
+            log("reference cpu tracer: %.2f Mrays/s on 1 thread" % cpu_ref["value"])
+        c1 = cpu_c1(threads)
+
+    fr = fr or {}
+    out = {
+        "metric": METRIC, "value": fr.get("mrays"), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fr.get("ms"), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "rtow-final-scene level-%d spheres (%s triangles), %dx%d, %d spp, depth %d, seed 0"
+                               % (level, fr.get("triangles"), w, h, spp, depth),
+                   "image": [w, h], "spp": spp, "depth": depth, "triangles": fr.get("triangles"),
+                   "parallelism": "single-gpu"},
+        "samples_per_s": fr.get("samples_per_s"), "Msamples_per_s": fr.get("Msamples_per_s"),
+        "rays_per_frame": fr.get("rays"), "rays_per_sample": fr.get("rays_per_sample"),
+        "bvh": fr.get("bvh"), "roofline": fr.get("roofline"), "rmse": rmse,
+        "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_c1": c1, "c2": c2,
+        "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
+        "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
+        "interactive": inter, "two_level": two,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
