@@ -535,10 +535,86 @@ def diff_leg(reps):
     rec = {"workload": "rtow-final-scene level-4 spheres, 640x360, 64 spp, depth 8, L2 image loss",
            "forward_ms": round(f_ms, 3), "adjoint_ms": round(b_ms, 3), "ratio": round(b_ms / f_ms, 3),
            "parameters": nparam}
+    stream.synchronize()
+    g = dpar.download(np.float32, (len(types), 4)).astype(np.float64)
     dldi.free()
     dpar.free()
+
+    def loss_of(m):
+        sc.set_materials(m)
+        la.ignite()
+        return float(((la.raw_rgb().astype(np.float64) - T) ** 2).sum())
+
+    # gradients against central differences (h = 1e-3) of the GPU render:
+    # albedo is exact (a path's structure does not depend on it); fuzz moves
+    # directions, and the render's differences then also hold the jumps of
+    # paths that cross a facet edge, a silhouette or the metal's absorption
+    # boundary, which the adjoint's interior derivative leaves out
+    hfd = 1e-3
+    for name, col, pool in (("albedo", None, [k for k, t in enumerate(types) if t in (TYPE_DIFFUSE, TYPE_REFLECT)]),
+                            ("fuzz", 3, [k for k, t in enumerate(types) if t == TYPE_REFLECT and m0[k, 3] >= 2 * hfd])):
+        cand = sorted(((abs(g[k, c]), k, c) for k in pool for c in ((0, 1, 2) if col is None else (col,))),
+                      reverse=True)[:8]
+        rels = []
+        for _, k, c in cand:
+            mp, mm = m0.copy(), m0.copy()
+            mp[k, c] = np.float32(m0[k, c] + hfd)
+            mm[k, c] = np.float32(m0[k, c] - hfd)
+            fd = (loss_of(mp) - loss_of(mm)) / (float(mp[k, c]) - float(mm[k, c]))
+            rels.append(abs(g[k, c] - fd) / abs(fd))
+        rec["%s_vs_render_fd" % name] = {"h": hfd, "params": len(rels), "median_rel_err": float(np.median(rels)),
+                                         "max_rel_err": float(max(rels))}
+        log("diff: %s adjoint vs render differences: median rel err %.2e, max %.2e"
+            % (name, np.median(rels), max(rels)))
+    sc.set_materials(m0)
+    rec["recovery"] = recover_leg(sc, la, lp, types, m0, mt, T)
     la.free()
     sc.close()
+    return rec
+
+
+def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.001):
+    """The recovery run of config 5 without torch: Adam on the L2 image loss
+    from albedo 0.5 and fuzz 0.25, gradients from rtx_launch_adjoint; the
+    mean absolute error of albedo and of fuzz over the parameters the image
+    constrains (nonzero gradient at the start), before and after."""
+    from rtxplay_amd import TYPE_DIFFUSE, TYPE_REFLECT
+    mask = np.zeros_like(m0)
+    for k, t in enumerate(types):
+        if t in (TYPE_DIFFUSE, TYPE_REFLECT):
+            mask[k, :3] = 1
+        if t == TYPE_REFLECT:
+            mask[k, 3] = 1
+    q = np.where(mask > 0, np.where(np.arange(4)[None, :] < 3, .5, .25), m0).astype(np.float64)
+    m, v = np.zeros_like(q), np.zeros_like(q)
+    t0 = time.perf_counter()
+    seen, loss0, loss = None, None, None
+    for it in range(iters + 1):
+        sc.set_materials(q.astype(np.float32))
+        la.ignite()
+        img = la.raw_rgb().astype(np.float64)
+        loss = float(((img - T) ** 2).sum())
+        g = la.adjoint((2.0 * (img - T)).astype(np.float32)).astype(np.float64) * mask
+        if it == 0:
+            loss0 = loss
+            seen = (np.abs(g) > 0) & (mask > 0)
+            e0 = np.abs(q - truth)
+        if it == iters:
+            break
+        a = lr * (lr_end / lr) ** (it / max(iters - 1, 1))
+        m = .9 * m + .1 * g
+        v = .999 * v + .001 * g * g
+        q = np.clip(q - a * (m / (1 - .9 ** (it + 1))) / (np.sqrt(v / (1 - .999 ** (it + 1))) + 1e-8), 0.0, 1.0)
+    e1 = np.abs(q - truth)
+    alb, fz = seen.copy(), seen.copy()
+    alb[:, 3] = False
+    fz[:, :3] = False
+    rec = {"iters": iters, "seconds": round(time.perf_counter() - t0, 1), "loss_start": round(loss0, 3),
+           "loss_end": round(loss, 3), "albedo_params": int(alb.sum()), "fuzz_params": int(fz.sum()),
+           "albedo_abs_err_start": round(float(e0[alb].mean()), 4), "albedo_abs_err_end": round(float(e1[alb].mean()), 4),
+           "fuzz_abs_err_start": round(float(e0[fz].mean()), 4), "fuzz_abs_err_end": round(float(e1[fz].mean()), 4)}
+    log("diff: recovery %s" % rec)
+    sc.set_materials(m0)
     return rec
 
 

@@ -560,7 +560,7 @@ static rtx_status build_scene_flat(rtx_ctx* c, hipStream_t s, Built** out) {
     b->hdr.xinst = nullptr;
     b->hdr.root = bvh.root;
     b->hdr.num_prims = (uint32_t)N;
-    b->hdr.num_nodes = bvh.num_nodes;
+    b->hdr.num_nodes = bvh.num_wide;
     b->hdr.num_insts = ninst;
     if ((e = hipMalloc((void**)&b->d_hdr, sizeof(rtx::DevScene))) != hipSuccess) return bail(hip_fail(e, "hipMalloc(handle)"));
     if ((e = hipMemcpyAsync(b->d_hdr, &b->hdr, sizeof(rtx::DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return bail(hip_fail(e, "upload handle"));

@@ -89,7 +89,7 @@ struct DevScene {
     const float4* xinst;  // two-level scenes: instance transforms and roots; nullptr: flat scene
     uint32_t root;        // inner node 0, a leaf ref, or kEmptyRoot
     uint32_t num_prims;
-    uint32_t num_nodes;
+    uint32_t num_nodes;   // 4-wide nodes in the node array
     uint32_t num_insts;
 };
 

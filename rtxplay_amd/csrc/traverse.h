@@ -127,6 +127,12 @@ __device__ __forceinline__ bool watertight(const TRay& r, float3 v0, float3 v1, 
 // LDS addresses as such (address space 3), so that stack accesses compile to
 // ds_read/ds_write instead of flat loads through a generic pointer
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) float4 lds_f4;
+typedef __attribute__((address_space(3))) const rtx_f4v lds_f4v;
+__device__ __forceinline__ float4 lld4(const char __attribute__((address_space(3)))* p) {
+    const rtx_f4v v = *(lds_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 template <int BLOCK, int SLDS = kStackLDS>
 struct LaneStack {
@@ -325,10 +331,14 @@ __device__ __forceinline__ bool first_active_lane() {
 // FMA selects the slab form: one FMA per plane with the per-ray offsets
 // (fewer instructions), or (P - o) * idir (six fewer registers, for kernels
 // that are short of them).
-template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, bool G = false, class Stack>
+// TOPN > 0: the first TOPN nodes of the node array (the top levels: the
+// array is breadth-first) are read from a copy in LDS at `top` instead of
+// global memory.
+template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, bool G = false, int TOPN = 0, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
                                          int inner_frac, Stack& st, uint32_t* nn = nullptr,
-                                         uint32_t* nt = nullptr, uint32_t* wi = nullptr) {
+                                         uint32_t* nt = nullptr, uint32_t* wi = nullptr,
+                                         const lds_f4* top = nullptr) {
     if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
         // node phase
@@ -374,10 +384,26 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             const char* nb = (const char*)sc.nodes;
             const uint32_t no = s.node * (16u * kWideNodeF4);
             const TRay& r = s.r;
-            const float4 ex = ld4<G>((cf4*)(nb + (no + r.nx))), fx = ld4<G>((cf4*)(nb + (no + (r.nx ^ 16u))));
-            const float4 ey = ld4<G>((cf4*)(nb + (no + r.ny))), fy = ld4<G>((cf4*)(nb + (no + (r.ny ^ 16u))));
-            const float4 ez = ld4<G>((cf4*)(nb + (no + r.nz))), fz = ld4<G>((cf4*)(nb + (no + (r.nz ^ 16u))));
-            const float4 rf = ld4<G>((cf4*)(nb + (no + 96u)));
+            float4 ex, fx, ey, fy, ez, fz, rf;
+            if (TOPN > 0 && s.node < (uint32_t)TOPN) {
+                typedef const char __attribute__((address_space(3))) lchar;
+                lchar* lb = (lchar*)top;
+                ex = lld4(lb + (no + r.nx));
+                fx = lld4(lb + (no + (r.nx ^ 16u)));
+                ey = lld4(lb + (no + r.ny));
+                fy = lld4(lb + (no + (r.ny ^ 16u)));
+                ez = lld4(lb + (no + r.nz));
+                fz = lld4(lb + (no + (r.nz ^ 16u)));
+                rf = lld4(lb + (no + 96u));
+            } else {
+                ex = ld4<G>((cf4*)(nb + (no + r.nx)));
+                fx = ld4<G>((cf4*)(nb + (no + (r.nx ^ 16u))));
+                ey = ld4<G>((cf4*)(nb + (no + r.ny)));
+                fy = ld4<G>((cf4*)(nb + (no + (r.ny ^ 16u))));
+                ez = ld4<G>((cf4*)(nb + (no + r.nz)));
+                fz = ld4<G>((cf4*)(nb + (no + (r.nz ^ 16u))));
+                rf = ld4<G>((cf4*)(nb + (no + 96u)));
+            }
             float key[4];
             uint32_t ref[4];
             int nhit = 0;

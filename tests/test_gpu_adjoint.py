@@ -159,6 +159,42 @@ def test_albedo_gradient_matches_finite_differences(c5, rtx):
     say("worst albedo rel err %.2e" % worst)
 
 
+def test_fuzz_gradient_matches_frozen_differences_on_the_c5_frame(c5, rtx, orc):
+    """The full config-5 frame: the 3 metals with the largest fuzz
+    gradients against central differences (h = 1e-3) of the oracle's
+    frozen-structure render, the interior derivative the adjoint computes
+    (the jumps of paths that cross a facet edge, a silhouette or the
+    absorption boundary are not in it; the benchmark reports their size
+    against differences of the real render).  The loss is taken over the
+    pixels where the GPU image and the oracle's agree."""
+    from rtxplay_amd import TYPE_REFLECT
+    meshes = {4: orc.sphere_mesh(1.0, 4)}
+    tris, inst = orc.flatten(meshes, c5.sc.things_spec, [4] * len(c5.types))
+    sc = orc.Scene(tris, inst, c5.sc.things_spec)
+    cam = orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, c5.w / c5.h, 0.1, 10.0)
+    say("oracle frozen render of the C5 frame")
+    agree = np.all(np.abs(sc.render_frozen(cam, c5.w, c5.h, c5.spp, c5.depth, c5.m0) - c5.I) <= 1e-5, axis=2)
+    say("pixels where GPU and oracle agree: %.4f" % agree.mean())
+    assert agree.mean() > 0.99
+    M = agree[..., None].astype(np.float64)
+    g = _adjoint(c5, c5.I, c5.T + (1.0 - M) * (c5.I - c5.T))
+    # central differences need fuzz >= h (the big metal sphere's fuzz is 0)
+    metals = sorted([k for k, t in enumerate(c5.types) if t == TYPE_REFLECT and c5.m0[k, 3] >= 2 * H_FD],
+                    key=lambda k: -abs(g[k, 3]))
+
+    def loss(m):
+        return float((M * (sc.render_frozen(cam, c5.w, c5.h, c5.spp, c5.depth, m) - c5.T) ** 2).sum())
+
+    for k in metals[:3]:
+        mp, mm = c5.m0.copy(), c5.m0.copy()
+        mp[k, 3] = np.float32(c5.m0[k, 3] + H_FD)
+        mm[k, 3] = np.float32(c5.m0[k, 3] - H_FD)
+        fd = (loss(mp) - loss(mm)) / (float(mp[k, 3]) - float(mm[k, 3]))
+        rel = abs(g[k, 3] - fd) / max(abs(fd), 1e-12)
+        say("C5 fuzz", k, "adjoint %.6e frozen fd %.6e rel %.2e" % (g[k, 3], fd, rel))
+        assert rel <= REL, (k, g[k, 3], fd)
+
+
 def test_albedo_gradient_iterative_semantics(rtx):
     """Under the iterative programs' semantics a sample that reaches the
     depth limit keeps its throughput (a product of albedos): at depth 3 most
