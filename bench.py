@@ -59,7 +59,7 @@ VALU_PEAK_GINST = CUS * PEAK_GHZ
 # per ray for its own input/output (ray 32 in, hit 16 out)
 NODE_BYTES, TRI_BYTES, SHADE_BYTES, IO_BYTES = 112, 48, 80, 48
 PROFILES = os.path.join(ROOT, "profiles")
-FRAME_KERNEL = "k_render<256, false, 32, false>"
+FRAME_KERNEL = "k_render<256, false, 32, false,"   # the frame kernel (either top-node staging)
 COPY_KERNEL = "k_pp<false>"
 # counter passes of the headline frame (each within one pass's block limits)
 PASSES = [
@@ -310,6 +310,9 @@ def counters_mode(args):
             if line.startswith("{"):
                 rays = json.loads(line).get("rays_per_frame")
         m, n = _counter_means(d, FRAME_KERNEL)
+        missing = [c for c in counters if c not in m]
+        if missing:
+            raise RuntimeError("pass %s: no values of %s for a kernel named %s*" % (name, missing, FRAME_KERNEL))
         res.update(m)
     cf = {}
     for name, ctr in (("copy_fetch", "FETCH_SIZE"), ("copy_write", "WRITE_SIZE")):
@@ -568,16 +571,20 @@ def diff_leg(reps):
             % (name, np.median(rels), max(rels)))
     sc.set_materials(m0)
     rec["recovery"] = recover_leg(sc, la, lp, types, m0, mt, T)
+    rec["recovery_fuzz_spsa"] = recover_leg(sc, la, lp, types, m0, mt, T, fuzz_spsa=0.02)
     la.free()
     sc.close()
     return rec
 
 
-def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.001):
+def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.001, fuzz_spsa=0.0, seed=11):
     """The recovery run of config 5 without torch: Adam on the L2 image loss
     from albedo 0.5 and fuzz 0.25, gradients from rtx_launch_adjoint; the
     mean absolute error of albedo and of fuzz over the parameters the image
-    constrains (nonzero gradient at the start), before and after."""
+    constrains (nonzero gradient at the start), before and after.  With
+    fuzz_spsa = c > 0 the fuzz gradient comes instead from simultaneous
+    perturbation of all fuzz values by +-c (two more renders per step), a
+    difference of the real render that holds the boundary terms."""
     from rtxplay_amd import TYPE_DIFFUSE, TYPE_REFLECT
     mask = np.zeros_like(m0)
     for k, t in enumerate(types):
@@ -587,6 +594,13 @@ def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.00
             mask[k, 3] = 1
     q = np.where(mask > 0, np.where(np.arange(4)[None, :] < 3, .5, .25), m0).astype(np.float64)
     m, v = np.zeros_like(q), np.zeros_like(q)
+    rng = np.random.default_rng(seed)
+
+    def lossq(p):
+        sc.set_materials(p.astype(np.float32))
+        la.ignite()
+        return float(((la.raw_rgb().astype(np.float64) - T) ** 2).sum())
+
     t0 = time.perf_counter()
     seen, loss0, loss = None, None, None
     for it in range(iters + 1):
@@ -596,8 +610,19 @@ def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.00
         loss = float(((img - T) ** 2).sum())
         g = la.adjoint((2.0 * (img - T)).astype(np.float32)).astype(np.float64) * mask
         if it == 0:
-            loss0 = loss
             seen = (np.abs(g) > 0) & (mask > 0)
+        if fuzz_spsa > 0:
+            fz = mask[:, 3] > 0
+            delta = np.where(rng.random(len(q)) < .5, -1.0, 1.0) * fz
+            lo = q.copy()
+            hi = q.copy()
+            hi[:, 3] = q[:, 3] + fuzz_spsa * delta
+            lo[:, 3] = q[:, 3] - fuzz_spsa * delta
+            lh = lossq(hi)
+            ll = lossq(lo)
+            g[:, 3] = np.where(fz, (lh - ll) / (2 * fuzz_spsa * np.where(fz, delta, 1.0)), 0.0)
+        if it == 0:
+            loss0 = loss
             e0 = np.abs(q - truth)
         if it == iters:
             break

@@ -10,6 +10,7 @@ path-replay kernel through the torch.autograd harness.  Both renders use the
 same per-pixel random streams.
 
     python -m rtxplay_amd.apps.recover_materials [--iters 200] [--w 640 --h 360 --spp 64 --depth 8]
+                                                 [--fuzz-spsa 0.02]
 
 Prints one line per report interval and a JSON summary: loss, and the mean
 absolute parameter error over the parameters the image constrains (those
@@ -41,6 +42,10 @@ def main(argv=None):
     ap.add_argument("--lr-end", type=float, default=0.001, help="learning rate at the last iteration (geometric decay)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--report", type=int, default=20)
+    ap.add_argument("--fuzz-spsa", type=float, default=0.0,
+                    help="c > 0: fuzz gradients from simultaneous perturbation of all fuzz values by +-c (two "
+                         "more renders per step, differences of the real render, so with the boundary terms "
+                         "the adjoint's interior derivative leaves out) instead of the adjoint")
     a = ap.parse_args(argv)
 
     sc = RTOWScene(seed=0, half=11, level=a.level)
@@ -92,6 +97,15 @@ def main(argv=None):
         opt.zero_grad()
         loss = ((r(q) - target) ** 2).sum()
         loss.backward()
+        if a.fuzz_spsa > 0:
+            with torch.no_grad():
+                fz = mask_t[:, 3] > 0
+                delta = torch.where(torch.rand(q.shape[0], device=dev) < .5, -1.0, 1.0) * fz
+                hi, lo = q.detach().clone(), q.detach().clone()
+                hi[:, 3] += a.fuzz_spsa * delta
+                lo[:, 3] -= a.fuzz_spsa * delta
+                d = float(((r.forward(hi) - target) ** 2).sum() - ((r.forward(lo) - target) ** 2).sum())
+                q.grad[:, 3] = torch.where(fz, d / (2 * a.fuzz_spsa * torch.where(fz, delta, 1.0)), 0.0)
         q.grad.mul_(mask_t)
         opt.step()
         sched.step()
@@ -104,7 +118,15 @@ def main(argv=None):
                   file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     qn = q.detach().cpu().numpy()
+    alb, fzm = seen.copy(), seen.copy()
+    alb[:, 3] = False
+    fzm[:, :3] = False
     out = {"iters": a.iters, "image": [a.w, a.h], "spp": a.spp, "depth": a.depth, "level": a.level,
+           "fuzz_spsa": a.fuzz_spsa,
+           "albedo_abs_err_start": float(np.abs(init - truth)[alb].mean()),
+           "albedo_abs_err_end": float(np.abs(qn - truth)[alb].mean()),
+           "fuzz_abs_err_start": float(np.abs(init - truth)[fzm].mean()) if fzm.any() else None,
+           "fuzz_abs_err_end": float(np.abs(qn - truth)[fzm].mean()) if fzm.any() else None,
            "parameters": int(mask.sum()), "constrained": int(seen.sum()),
            "loss_start": float(loss0.detach()), "loss_end": float(loss.detach()),
            "mean_abs_err_start": err0, "mean_abs_err_end": float(np.abs(qn - truth)[seen].mean()),

@@ -446,37 +446,44 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
 // nodes of the top levels that each block of the frame kernels copies to LDS
 // (0: none; the node array is breadth-first, so these are nodes 0..N-1).
 // 64 nodes (8 KB: the top three 4-wide levels and 43 of the fourth) with the
-// 32-entry LDS stack fill the CU's 160 KB at 4 blocks per CU; measured on
-// the 1080p / 500 spp frame: 21 nodes -0.5 %, 42 -0.4 %, 64 -1.1 % time
+// 32-entry LDS stack fill the CU's 160 KB at 4 blocks per CU.  Measured: on
+// the 1080p / 500 spp frame of the level-6 scene (2.3 M nodes) 21 nodes
+// -0.5 %, 42 -0.4 %, 64 -1.1 % time; on scenes whose arrays fit the MALL
+// (the level-4 scene, 0.15 M nodes: +2.5 %; the batch's small scenes: +4 %)
+// and on short frames of the level-6 scene (1 spp: +3 %) the waves whose
+// lanes split between LDS and global nodes issue both loads for too little
+// latency saved.  So frames of at least kOrderMinSpp samples per pixel of
+// scenes of at least kLdsTopMinNodes nodes take the staged kernels.
 #ifndef RTX_LDS_TOP
 #define RTX_LDS_TOP 64
 #endif
 constexpr int kLdsTop = RTX_LDS_TOP;
+constexpr uint32_t kLdsTopMinNodes = 1u << 20;   // 128 MB of nodes
 
-// copies the scene's first kLdsTop nodes (fewer if it has fewer) to s_top
-template <int BLOCK>
+// copies the scene's first TOPN nodes (fewer if it has fewer) to s_top
+template <int BLOCK, int TOPN>
 __device__ __forceinline__ void stage_top(const DevScene& sc, float4* s_top) {
-    const uint32_t n = (sc.num_nodes < (uint32_t)kLdsTop ? sc.num_nodes : (uint32_t)kLdsTop) * (uint32_t)kWideNodeF4;
+    const uint32_t n = (sc.num_nodes < (uint32_t)TOPN ? sc.num_nodes : (uint32_t)TOPN) * (uint32_t)kWideNodeF4;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) s_top[i] = sc.nodes[i];
     __syncthreads();
 }
 
 // single frame: one block of 256 lanes per SIMD wave slot
-template <int BLOCK, bool COUNT, int SLDS, bool TL>
+template <int BLOCK, bool COUNT, int SLDS, bool TL, int TOPN>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_render(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    __shared__ float4 s_top[kLdsTop > 0 ? kLdsTop * kWideNodeF4 : 1];
-    if (kLdsTop > 0) stage_top<BLOCK>(a.job.sc, s_top);
-    render_body<BLOCK, COUNT, false, SLDS, false, TL, true, false, kLdsTop>(a, s_stack, nullptr, (const lds_f4*)s_top);
+    __shared__ float4 s_top[TOPN > 0 ? TOPN * kWideNodeF4 : 1];
+    if (TOPN > 0) stage_top<BLOCK, TOPN>(a.job.sc, s_top);
+    render_body<BLOCK, COUNT, false, SLDS, false, TL, true, false, TOPN>(a, s_stack, nullptr, (const lds_f4*)s_top);
 }
 
 // the probe of a plain frame: one sample per pixel, rays counted per tile
-template <int BLOCK, int SLDS, bool TL>
+template <int BLOCK, int SLDS, bool TL, int TOPN>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDER_WAVES))) void k_probe(const RenderArgs a) {
     __shared__ uint32_t s_stack[SLDS * BLOCK];
-    __shared__ float4 s_top[kLdsTop > 0 ? kLdsTop * kWideNodeF4 : 1];
-    if (kLdsTop > 0) stage_top<BLOCK>(a.job.sc, s_top);
-    render_body<BLOCK, false, false, SLDS, false, TL, true, true, kLdsTop>(a, s_stack, nullptr, (const lds_f4*)s_top);
+    __shared__ float4 s_top[TOPN > 0 ? TOPN * kWideNodeF4 : 1];
+    if (TOPN > 0) stage_top<BLOCK, TOPN>(a.job.sc, s_top);
+    render_body<BLOCK, false, false, SLDS, false, TL, true, true, TOPN>(a, s_stack, nullptr, (const lds_f4*)s_top);
 }
 
 __global__ void k_zero_n(uint32_t* p, uint32_t n) {
@@ -641,16 +648,28 @@ uint32_t log2u(uint32_t v) {
 }
 
 // the frame kernel for the scene's kind (flat / two-level) and the LDS stack size
-template <bool COUNT>
-hipError_t launch_frame(const RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
+// the frame kernel for the scene's kind (flat / two-level), the LDS stack
+// size and whether the top nodes go to LDS (large scenes, not the counted launch)
+template <bool COUNT, int TOPN>
+void launch_frame_t(const RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
     const dim3 g(ls.grid), b(kRenderBlock);
     if (a.job.sc.xinst) {
-        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, true>), g, b, 0, s, a);
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, true, TOPN>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, true, TOPN>), g, b, 0, s, a);
     } else {
-        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, false>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, false>), g, b, 0, s, a);
+        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 16, false, TOPN>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_render<kRenderBlock, COUNT, 32, false, TOPN>), g, b, 0, s, a);
     }
+}
+
+bool stage_top_nodes(const Job& j) {
+    return kLdsTop > 0 && j.sc.num_nodes >= kLdsTopMinNodes && j.lp.spp >= kOrderMinSpp;
+}
+
+template <bool COUNT>
+hipError_t launch_frame(const RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
+    if (!COUNT && stage_top_nodes(a.job)) launch_frame_t<COUNT, kLdsTop>(a, ls, s);
+    else launch_frame_t<COUNT, 0>(a, ls, s);
     return hipGetLastError();
 }
 
@@ -685,7 +704,7 @@ Job make_job(const DevScene& sc, const rtx_lp_general& lp, float3 ambient, uint3
 
 uint32_t render_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false, kStackLDS, false>, kRenderBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kRenderBlock, false, kStackLDS, false, kLdsTop>, kRenderBlock, 0) !=
             hipSuccess || nb < 1)
         nb = 1;
     return (uint32_t)nb;
@@ -707,12 +726,22 @@ void order_tiles(RenderArgs& a, const LaunchScratch& ls, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_n, dim3((ntiles + 255u) / 256u), dim3(256), 0, s, ls.tile_cost, ntiles);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s, ls.counter);
     const dim3 g(ls.grid), b(kRenderBlock);
-    if (a.job.sc.xinst) {
-        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true>), g, b, 0, s, p);
+    if (stage_top_nodes(a.job)) {
+        if (a.job.sc.xinst) {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true, kLdsTop>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true, kLdsTop>), g, b, 0, s, p);
+        } else {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false, kLdsTop>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false, kLdsTop>), g, b, 0, s, p);
+        }
     } else {
-        if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false>), g, b, 0, s, p);
+        if (a.job.sc.xinst) {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, true, 0>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, true, 0>), g, b, 0, s, p);
+        } else {
+            if (ls.stack_lds == 16) hipLaunchKernelGGL((k_probe<kRenderBlock, 16, false, 0>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_probe<kRenderBlock, 32, false, 0>), g, b, 0, s, p);
+        }
     }
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBlock), 0, s, ls.tile_cost, ntiles, 64u * a.job.lp.depth,
                        ls.tile_order);
