@@ -321,6 +321,14 @@ def counters_mode(args):
         cf[ctr] = m[ctr] * n[ctr]                 # KiB over all the probe's launches
     d, _ = run("trace", ["--kernel-trace", "--stats"], frame[:5] + ["2"] + frame[6:])
     stats = _kernel_stats(d)
+    res_kernel = {}
+    tf = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    for r in (_csv_rows(tf[0]) if tf else []):
+        if FRAME_KERNEL in r.get("Kernel_Name", ""):
+            res_kernel = {"lds_bytes_per_block": int(r["LDS_Block_Size"]),
+                          "workgroup": int(r["Workgroup_Size_X"]), "grid": int(r["Grid_Size_X"]),
+                          "name": r["Kernel_Name"]}
+            break
     npx = 64 << 20
     read_factor = 3 * 12 * npx / (cf["FETCH_SIZE"] * 1024.0)     # known bytes / counted bytes
     write_factor = 3 * 4 * npx / (cf["WRITE_SIZE"] * 1024.0)
@@ -330,7 +338,7 @@ def counters_mode(args):
     summary = {"workload": args.workload, "kernel": FRAME_KERNEL, "stamp": kernel_stamp(), "rays_per_launch": rays,
                "counters_per_launch": res, "copy_probe_kib": cf, "read_factor": read_factor,
                "write_factor": write_factor, "hbm_bytes_per_launch": hbm, "kernel_avg_us": avg_us,
-               "commands": cmds}
+               "kernel_resources": res_kernel, "commands": cmds}
     jpath = os.path.join(out, "k_render_%s_counters.json" % args.workload)
     json.dump(summary, open(jpath, "w"), indent=1, sort_keys=True)
     lines = ["# counters of the headline frame kernel %s, workload %s (kernel sources stamp %s)"
@@ -357,6 +365,10 @@ def counters_mode(args):
                  % (s["TCC_HIT_sum"] / (s["TCC_HIT_sum"] + s["TCC_MISS_sum"]),
                     s["TCP_TCC_READ_REQ_sum"] / s["TCP_TOTAL_CACHE_ACCESSES_sum"]))
     lines.append("kernel trace: %s average %.1f us over the timed frames" % (FRAME_KERNEL, avg_us or float("nan")))
+    if res_kernel:
+        lines.append("launch: %s: %d bytes of LDS per block of %d lanes, grid of %d lanes"
+                     % (res_kernel["name"][:80], res_kernel["lds_bytes_per_block"], res_kernel["workgroup"],
+                        res_kernel["grid"]))
     lines.append("")
     lines.append("kernel trace summary (rocprofv3 --kernel-trace --stats, 2 frames):")
     keys = list(stats[0].keys()) if stats else []
