@@ -97,10 +97,21 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
     return lo;
 }
 
+// the render kernels test parked leaves with the pairs spread over the wave
+// (leaves_compact in traverse.h; measured on the 1080p / 500 spp level-6
+// frame: triangle-loop lanes busy 27 % -> about 60 %, frame -3 %); 0 keeps
+// the per-lane loop
+#ifndef RTX_COMPACT_LEAVES
+#define RTX_COMPACT_LEAVES 1
+#endif
+constexpr bool kCompactLeaves = RTX_COMPACT_LEAVES != 0;
+
 template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false, bool FMA = true,
           bool PROBE = false, int TOPN = 0>
 __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr,
                                             const lds_f4* top = nullptr) {
+    __shared__ uint32_t s_cl[kCompactLeaves ? (BLOCK / 64) * kLeafScratchWords : 1];
+    lds_u32* cl = (lds_u32*)(s_cl + (kCompactLeaves ? (threadIdx.x >> 6) * kLeafScratchWords : 0u));
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = a.gstack + blockIdx.x * BLOCK + threadIdx.x;
@@ -231,8 +242,9 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT, TL, FMA, BATCH, TOPN>(sc, tv, going, active, (int)a.batch, (int)a.inner_frac, stk,
-                                                     &cnt_nodes, &cnt_tris, wave_iters, top);
+        trav_run<BLOCK, COUNT, TL, FMA, BATCH, TOPN, kCompactLeaves>(sc, tv, going, active, (int)a.batch,
+                                                                     (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
+                                                                     wave_iters, top, cl);
         if (COUNT) cyc_trav += clock64() - c0;
         if (!active || going) continue;
 
@@ -445,17 +457,19 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
 
 // nodes of the top levels that each block of the frame kernels copies to LDS
 // (0: none; the node array is breadth-first, so these are nodes 0..N-1).
-// 64 nodes (8 KB: the top three 4-wide levels and 43 of the fourth) with the
-// 32-entry LDS stack fill the CU's 160 KB at 4 blocks per CU.  Measured: on
-// the 1080p / 500 spp frame of the level-6 scene (2.3 M nodes) 21 nodes
-// -0.5 %, 42 -0.4 %, 64 -1.1 % time; on scenes whose arrays fit the MALL
+// With the 32-entry LDS stack (32 KB per block) and the compacted leaf
+// phase's scratch (4 KB per block), 32 nodes (3.5 KB: the top two 4-wide
+// levels and 11 of the third) fill the CU's 160 KB at 4 blocks per CU.
+// Measured: on the 1080p / 500 spp frame of the level-6 scene (2.3 M nodes),
+// before the leaf phase was compacted, 21 nodes -0.5 %, 42 -0.4 %, 64 -1.1 %
+// time, and with it 32 nodes -0.8 %; on scenes whose arrays fit the MALL
 // (the level-4 scene, 0.15 M nodes: +2.5 %; the batch's small scenes: +4 %)
 // and on short frames of the level-6 scene (1 spp: +3 %) the waves whose
 // lanes split between LDS and global nodes issue both loads for too little
 // latency saved.  So frames of at least kOrderMinSpp samples per pixel of
 // scenes of at least kLdsTopMinNodes nodes take the staged kernels.
 #ifndef RTX_LDS_TOP
-#define RTX_LDS_TOP 64
+#define RTX_LDS_TOP 32
 #endif
 constexpr int kLdsTop = RTX_LDS_TOP;
 constexpr uint32_t kLdsTopMinNodes = 1u << 20;   // 128 MB of nodes

@@ -313,6 +313,134 @@ __device__ __forceinline__ bool first_active_lane() {
     return (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)e) - 1);
 }
 
+// LDS words per wave that the compacted leaf phase uses: the inclusive
+// prefix of the pair counts (64), the best key per lane (64 x 64 bit) and
+// the lane holding it (64)
+constexpr uint32_t kLeafScratchWords = 256;
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+
+// orders the LDS accesses of the wave's lanes around it (the lanes of a wave
+// run in lockstep and its LDS operations complete in order, so this only
+// keeps the compiler from moving them)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// The leaf phase with the work spread over the wave.  A lane's parked
+// leaves hold c = c1 + c2 triangles; the wave's (lane, triangle) pairs are
+// numbered by an inclusive prefix sum of c and dealt out 64 at a time, one
+// per lane whatever lane parked them.  The lane that tests pair p finds its
+// owner (the first lane whose prefix exceeds p) by a binary search of the
+// prefix array in LDS and reads the owner's ray and leaves with
+// ds_bpermute.  A hit goes into the owner's LDS slot as the 64-bit minimum
+// of (t, global primitive id) -- t > tmin >= 0, so the float's bits order
+// like its value -- and the lane whose key won writes its lane number next
+// to it; the owner then takes that hit (t, u, v, triangle) under the rule of
+// the per-lane loop: closer, or as close with the lower id.  The minimum of
+// (t, id) does not depend on the order of the tests, so the closest hit is
+// the one the lane would have found testing its own leaves in order.
+// Needs all 64 lanes of the wave in the call (the pairs of an inactive lane
+// would never be tested), and a ray with tmin >= 0.  A pair is tested only
+// if it lies in its owner's own leaves (q < the owner's count), and read
+// from the owner's triangle array, so every triangle read is one the owner
+// parked.
+template <bool G, bool TL, bool COUNT>
+__device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint32_t f1, uint32_t c1, uint32_t f2,
+                                               uint32_t c2, lds_u32* cl, uint32_t* wi) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = c1 + c2;
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, (unsigned)d, 64);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    if (total == 0) return;
+    const uint32_t excl = incl - c;
+    lds_u32* B = cl;
+    lds_u64* K = (lds_u64*)(cl + 64);
+    lds_u32* Wn = cl + 192;
+    B[lane] = incl;
+    K[lane] = ~0ull;
+    // what a helper lane needs of the owner: its ray (object space inside an
+    // instance), the first pair's number, and its leaves
+    const uint32_t meta = excl | (c1 << 16) | (c << 21) | (s.r.rz0 ? 1u << 30 : 0u) | (s.r.rz1 ? 1u << 31 : 0u);
+    wave_sync();
+    for (uint32_t base = 0; base < total; base += 64u) {
+        if (COUNT && lane == 0) wi[1]++;
+        const uint32_t p = base + lane;
+        const bool has = p < total;
+        // owner: the number of the first 63 prefixes that are <= p (B[63] = total > p)
+        uint32_t i = 0;
+#pragma unroll
+        for (uint32_t st = 32; st >= 1; st >>= 1)
+            if (B[i + st - 1u] <= p) i += st;
+        const int ad = (int)(i << 2);
+        auto bp = [&](uint32_t v) -> uint32_t { return (uint32_t)__builtin_amdgcn_ds_bpermute(ad, (int)v); };
+        auto bpf = [&](float v) -> float { return __uint_as_float(bp(__float_as_uint(v))); };
+        TRay hr;
+        hr.o = f3(bpf(s.r.o.x), bpf(s.r.o.y), bpf(s.r.o.z));
+        hr.Sx = bpf(s.r.Sx);
+        hr.Sy = bpf(s.r.Sy);
+        hr.Sz = bpf(s.r.Sz);
+        hr.tmin = bpf(s.r.tmin);
+        const uint32_t om = bp(meta), of1 = bp(f1), of2 = bp(f2);
+        hr.rz0 = (om >> 30) & 1u;
+        hr.rz1 = (om >> 31) & 1u;
+        const uint32_t q = p - (om & 0xFFFFu), oc1 = (om >> 16) & 31u, oc = (om >> 21) & 63u;
+        const uint32_t k = q < oc1 ? of1 + q : of2 + (q - oc1);
+        float tt = 0.f, uu = 0.f, vv = 0.f;
+        bool hit = false;
+        uint64_t key = ~0ull;
+        // the owner's triangle array: in a batch (G) each lane renders its
+        // own job, so the owner's scene need not be this lane's
+        const char* tb = (const char*)sc.tris;
+        if (G) {
+            const uint64_t pt = (uint64_t)sc.tris;
+            tb = (const char*)(((uint64_t)bp((uint32_t)(pt >> 32)) << 32) | (uint64_t)bp((uint32_t)pt));
+        }
+        if (has && q < oc) {
+            const uint32_t to = k * 48u;
+            typedef const float4 cf4t;
+            const float4 t0 = ld4<G>((cf4t*)(tb + to)), t1 = ld4<G>((cf4t*)(tb + (to + 16u))),
+                         t2 = ld4<G>((cf4t*)(tb + (to + 32u)));
+            hit = watertight(hr, xyz(t0), xyz(t1), xyz(t2), tt, uu, vv);
+            if (hit) {
+                key = ((uint64_t)__float_as_uint(tt) << 32) | (uint64_t)__float_as_uint(t0.w);
+                __hip_atomic_fetch_min(K + i, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        }
+        wave_sync();
+        if (hit && K[i] == key) Wn[i] = lane;
+        wave_sync();
+        // the owners of this round's pairs take their best hit
+        const uint64_t kk = K[lane];
+        const int wa = (int)((Wn[lane] & 63u) << 2);
+        const float bu = __uint_as_float((uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)__float_as_uint(uu)));
+        const float bv = __uint_as_float((uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)__float_as_uint(vv)));
+        const uint32_t bk = (uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)k);
+        const bool mine = c > 0 && excl < base + 64u && incl > base;
+        if (mine && kk != ~0ull) {
+            const float t = __uint_as_float((uint32_t)(kk >> 32));
+            const uint32_t prim = (uint32_t)kk;
+            const bool lower = TL ? (s.inst < s.h.inst || (s.inst == s.h.inst && prim < s.h.prim)) : prim < s.h.prim;
+            if (t < s.h.t || (t == s.h.t && lower)) {
+                s.h.t = t;
+                s.h.u = bu;
+                s.h.v = bv;
+                s.h.tri = bk;
+                s.h.prim = prim;
+                if (TL) s.h.inst = s.inst;
+            }
+            K[lane] = ~0ull;
+        }
+        wave_sync();
+    }
+}
+
 // Advances the traversal of the lanes with `going` set.  A lane clears
 // `going` when its ray is done (closest hit final in s.h).
 //
@@ -334,11 +462,15 @@ __device__ __forceinline__ bool first_active_lane() {
 // TOPN > 0: the first TOPN nodes of the node array (the top levels: the
 // array is breadth-first) are read from a copy in LDS at `top` instead of
 // global memory.
-template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, bool G = false, int TOPN = 0, class Stack>
+// CL: the leaf phase spreads the parked leaves' triangles over the wave
+// (leaves_compact, with the wave's kLeafScratchWords of LDS at `cl`);
+// otherwise each lane tests its own leaves.
+template <int BLOCK, bool COUNT = false, bool TL = false, bool FMA = true, bool G = false, int TOPN = 0,
+          bool CL = false, class Stack>
 __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& going, bool busy, int batch,
                                          int inner_frac, Stack& st, uint32_t* nn = nullptr,
                                          uint32_t* nt = nullptr, uint32_t* wi = nullptr,
-                                         const lds_f4* top = nullptr) {
+                                         const lds_f4* top = nullptr, lds_u32* cl = nullptr) {
     if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
         // node phase
@@ -453,8 +585,22 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 s.node = ref[0];
             }
         }
-        // leaf phase
-        if (going && s.leaf != kNoChild) {
+        // leaf phase (compacted when the whole wave is here)
+        if (CL && __builtin_amdgcn_read_exec() == ~0ull) {
+            uint32_t f1 = 0, c1 = 0, f2 = 0, c2 = 0;
+            if (going && s.leaf != kNoChild) {
+                f1 = s.leaf & kLeafFirstMask;
+                c1 = ((s.leaf >> kLeafCountShift) & 15u) + 1u;
+                if (s.leaf2 != kNoChild) {
+                    f2 = s.leaf2 & kLeafFirstMask;
+                    c2 = ((s.leaf2 >> kLeafCountShift) & 15u) + 1u;
+                }
+                s.leaf = kNoChild;
+                s.leaf2 = kNoChild;
+            }
+            if (COUNT) *nt += c1 + c2;
+            leaves_compact<G, TL, COUNT>(sc, s, f1, c1, f2, c2, cl, wi);
+        } else if (going && s.leaf != kNoChild) {
             const uint32_t lf = s.leaf;
             s.leaf = kNoChild;
             const uint32_t first = lf & kLeafFirstMask;
