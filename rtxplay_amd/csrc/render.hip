@@ -104,14 +104,22 @@ __device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, ui
 #ifndef RTX_COMPACT_LEAVES
 #define RTX_COMPACT_LEAVES 1
 #endif
+// the two-level frame kernels take the (P - o) * idir slab form: its six
+// fewer registers keep them from spilling next to the compacted leaf phase
+// (1080p / 64 spp: level-6 scene 98.7 -> 96.7 ms, mixed levels 114.1 ->
+// 109.8 ms against the FMA form)
+#ifndef RTX_TL_FMA
+#define RTX_TL_FMA 0
+#endif
 constexpr bool kCompactLeaves = RTX_COMPACT_LEAVES != 0;
 
 template <int BLOCK, bool COUNT, bool BATCH, int SLDS, bool ADJ = false, bool TL = false, bool FMA = true,
           bool PROBE = false, int TOPN = 0>
 __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_stack, float* s_grad = nullptr,
                                             const lds_f4* top = nullptr) {
-    __shared__ uint32_t s_cl[kCompactLeaves ? (BLOCK / 64) * kLeafScratchWords : 1];
-    lds_u32* cl = (lds_u32*)(s_cl + (kCompactLeaves ? (threadIdx.x >> 6) * kLeafScratchWords : 0u));
+    constexpr bool CL = kCompactLeaves;
+    __shared__ uint32_t s_cl[CL ? (BLOCK / 64) * kLeafScratchWords : 1];
+    lds_u32* cl = (lds_u32*)(s_cl + (CL ? (threadIdx.x >> 6) * kLeafScratchWords : 0u));
     LaneStack<BLOCK, SLDS> stk;
     stk.lds = (lds_u32*)(s_stack + threadIdx.x);
     stk.glob = a.gstack + blockIdx.x * BLOCK + threadIdx.x;
@@ -242,7 +250,7 @@ __device__ __forceinline__ void render_body(const RenderArgs& a, uint32_t* s_sta
         // traverse until a batch of lanes has a final closest hit; the others
         // keep their traversal state and resume after the shading below
         const uint64_t c0 = COUNT ? clock64() : 0;
-        trav_run<BLOCK, COUNT, TL, FMA, BATCH, TOPN, kCompactLeaves>(sc, tv, going, active, (int)a.batch,
+        trav_run<BLOCK, COUNT, TL, FMA, BATCH, TOPN, CL>(sc, tv, going, active, (int)a.batch,
                                                                      (int)a.inner_frac, stk, &cnt_nodes, &cnt_tris,
                                                                      wave_iters, top, cl);
         if (COUNT) cyc_trav += clock64() - c0;
@@ -488,7 +496,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTX_RENDE
     __shared__ uint32_t s_stack[SLDS * BLOCK];
     __shared__ float4 s_top[TOPN > 0 ? TOPN * kWideNodeF4 : 1];
     if (TOPN > 0) stage_top<BLOCK, TOPN>(a.job.sc, s_top);
-    render_body<BLOCK, COUNT, false, SLDS, false, TL, true, false, TOPN>(a, s_stack, nullptr, (const lds_f4*)s_top);
+    render_body<BLOCK, COUNT, false, SLDS, false, TL, !TL || RTX_TL_FMA, false, TOPN>(a, s_stack, nullptr,
+                                                                                   (const lds_f4*)s_top);
 }
 
 // the probe of a plain frame: one sample per pixel, rays counted per tile

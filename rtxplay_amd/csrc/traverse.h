@@ -422,8 +422,9 @@ __device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint
         const float bu = __uint_as_float((uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)__float_as_uint(uu)));
         const float bv = __uint_as_float((uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)__float_as_uint(vv)));
         const uint32_t bk = (uint32_t)__builtin_amdgcn_ds_bpermute(wa, (int)k);
-        const bool mine = c > 0 && excl < base + 64u && incl > base;
-        if (mine && kk != ~0ull) {
+        // (a slot holds a key only if its lane owned a pair of this round
+        // that hit: every slot is reset when it is taken)
+        if (kk != ~0ull) {
             const float t = __uint_as_float((uint32_t)(kk >> 32));
             const uint32_t prim = (uint32_t)kk;
             const bool lower = TL ? (s.inst < s.h.inst || (s.inst == s.h.inst && prim < s.h.prim)) : prim < s.h.prim;
@@ -473,7 +474,8 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                                          const lds_f4* top = nullptr, lds_u32* cl = nullptr) {
     if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
-        // node phase
+        // node phase (no lane starts or stops going in it)
+        const uint32_t n_going = (uint32_t)__popcll(__ballot(going));
         for (;;) {
             if (TL) {
                 // Instance transitions: leave a finished instance (not while
@@ -485,7 +487,7 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 const bool leave = going && s.node == kInstMarker && s.leaf == kNoChild;
                 const bool enter = going && (s.node & kInstLeaf) == kInstLeaf;
                 const uint64_t need = __ballot(leave || enter);
-                if (need && (uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * (uint32_t)__popcll(__ballot(going)))
+                if (need && (uint32_t)__popcll(need) * 64u >= (uint32_t)kTlFrac * n_going)
                     inst_step<G>(sc, s, going, leave, st);
             }
             // a lane that reached a leaf parks it (two slots) and goes on
@@ -501,7 +503,7 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
             const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild && (!TL || s.node != kInstMarker);
             const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
             if (n_inner == 0) break;
-            if (n_inner * 64u < (uint32_t)inner_frac * (uint32_t)__popcll(__ballot(going))) break;
+            if (n_inner * 64u < (uint32_t)inner_frac * n_going) break;
             if (!inner) continue;
             if (COUNT) {
                 (*nn)++;
