@@ -271,6 +271,15 @@ __device__ __forceinline__ float xf_point(const float4& row, float3 p) {
 __device__ __forceinline__ float xf_vector(const float4& row, float3 v) {
     return (v.x * row.x + v.y * row.y) + v.z * row.z;
 }
+// the same products as fused multiply-adds, for the ray an instance entry
+// transforms into object space: nothing has to round like the flat scene
+// there, and three FMAs per row replace six (five) operations
+__device__ __forceinline__ float xf_point_fma(const float4& row, float3 p) {
+    return __builtin_fmaf(p.x, row.x, __builtin_fmaf(p.y, row.y, __builtin_fmaf(p.z, row.z, row.w)));
+}
+__device__ __forceinline__ float xf_vector_fma(const float4& row, float3 v) {
+    return __builtin_fmaf(v.x, row.x, __builtin_fmaf(v.y, row.y, v.z * row.z));
+}
 
 // two-level scenes: one instance transition of the lanes that call it.  A
 // lane with `leave` set has just popped its instance's marker (and has no
@@ -299,8 +308,8 @@ __device__ __forceinline__ void inst_step(const DevScene& sc, Trav& s, bool goin
                                __float_as_uint(d.y), __float_as_uint(d.z), kInstMarker};
         st.push_n(s.sp, w);
         const float3 wo = o, wd = d;
-        o = f3(xf_point(r0, wo), xf_point(r1, wo), xf_point(r2, wo));
-        d = f3(xf_vector(r0, wd), xf_vector(r1, wd), xf_vector(r2, wd));
+        o = f3(xf_point_fma(r0, wo), xf_point_fma(r1, wo), xf_point_fma(r2, wo));
+        d = f3(xf_vector_fma(r0, wd), xf_vector_fma(r1, wd), xf_vector_fma(r2, wd));
         s.inst = id;
         s.node = __float_as_uint(m.x);
     }
