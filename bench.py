@@ -23,11 +23,11 @@ L1, L2 and the MALL, most of them never from HBM).
 
 cpu_baseline: the oracle's renderer (C, same scene, same semantics) on the
 host cores over a bounded sample of rows; cpu_reference: the reference's
-own rtow.cxx built from its sources; cpu_c1: the rtow.cxx tracer at config
-1 (400x225, 10 spp, 16 rays per sample) in FP64 as written and in FP32 on
-one and on all threads.
+own rtow.cxx built from its sources; cpu_rtow_400x225: the rtow.cxx tracer
+on its own analytic scene (400x225, 10 spp, 16 rays per sample) in FP64 as
+written and in FP32 on one and on all threads.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload c3|c2]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload level6-1080p|level4-720p]
     python bench.py --counters      # the counter profile (rocprofv3 passes)
 """
 import argparse
@@ -47,8 +47,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + samples/s at 1080p 500spp (RMSE-matched); batched speedup; bwd/fwd ratio"
 WORKLOADS = {
     # name: (level, w, h, spp, depth)
-    "c3": (6, 1920, 1080, 500, 16),
-    "c2": (4, 1280, 720, 50, 16),
+    "level6-1080p": (6, 1920, 1080, 500, 16),
+    "level4-720p": (4, 1280, 720, 50, 16),
 }
 HBM_PEAK_GBS = 8000.0
 # VALU issue: one wave instruction per CU per clock, 256 CUs at the 2.4 GHz peak clock
@@ -187,8 +187,8 @@ def cpu_reference(seconds):
     return rec
 
 
-def cpu_c1(threads):
-    """Config 1: rtow.cxx's tracer (analytic spheres, its own scene and
+def cpu_rtow_small(threads):
+    """rtow.cxx's tracer (analytic spheres, its own scene and
     camera) at 400x225, 10 spp, 16 rays per sample (trace depth 15), in
     three forms: FP64 on the reference's glibc stream, one thread (rtow.cxx
     as written); FP32 on per-pixel streams, one thread; FP32 on all host
@@ -211,7 +211,7 @@ def cpu_c1(threads):
                      "Msamples_per_s": round(reps * w * h * spp / dt / 1e6, 3),
                      "rays_per_sample": round(r["rays"] / (w * h * spp), 4),
                      "image_mean": [round(float(x), 5) for x in r["mean"].reshape(-1, 3).mean(axis=0)]}
-        log("c1 %s: %.2f Mrays/s (%.2f s)" % (name, rec[name]["Mrays_per_s"], dt))
+        log("rtow 400x225 %s: %.2f Mrays/s (%.2f s)" % (name, rec[name]["Mrays_per_s"], dt))
     return rec
 
 
@@ -590,7 +590,7 @@ def diff_leg(reps):
 
 
 def recover_leg(sc, la, lp, types, m0, truth, T, iters=150, lr=0.02, lr_end=0.001, fuzz_spsa=0.0, seed=11):
-    """The recovery run of config 5 without torch: Adam on the L2 image loss
+    """The recovery run of the differentiable frame without torch: Adam on the L2 image loss
     from albedo 0.5 and fuzz 0.25, gradients from rtx_launch_adjoint; the
     mean absolute error of albedo and of fuzz over the parameters the image
     constrains (nonzero gradient at the start), before and after.  With
@@ -829,13 +829,13 @@ def frame_leg(args, level, w, h, spp, depth):
             "roofline": rf}, raw
 
 
-def c2_leg(steps=5):
-    """Config 2: the level-4 scene (497,664 triangles), 1280x720, 50 spp,
+def level4_leg(steps=5):
+    """The level-4 scene (497,664 triangles), 1280x720, 50 spp,
     depth 16: build, one warmup, `steps` timed launches."""
     import rtxplay_amd as rtx
     from rtxplay_amd.hip import Event, Stream
     from rtxplay_amd.scene import RTOWScene
-    level, w, h, spp, depth = WORKLOADS["c2"]
+    level, w, h, spp, depth = WORKLOADS["level4-720p"]
     sc = RTOWScene(seed=0, half=11, level=level)
     sc.load()
     lp = rtx.LpGeneral()
@@ -860,7 +860,7 @@ def c2_leg(steps=5):
            "Mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
            "Msamples_per_s": round(w * h * spp / (ms * 1e-3) / 1e6, 1), "rays_per_frame": rays,
            "build_ms": round(st["build_ms"], 1)}
-    log("c2: %s" % rec)
+    log("level4: %s" % rec)
     return rec
 
 
@@ -869,14 +869,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="level6-1080p", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against the 4096-spp CPU reference")
     ap.add_argument("--stats-spp", type=int, default=16)
     ap.add_argument("--shade-batch", type=int, default=0, help="render tuning (0: the library default)")
-    ap.add_argument("--legs", default="frame,c2,batch,diff,interactive,two_level",
-                    help="comma list of: frame (the headline render), c2 (config 2), batch (64 frames), "
+    ap.add_argument("--legs", default="frame,level4,batch,diff,interactive,two_level",
+                    help="comma list of: frame (the headline render), level4 (the level-4 scene at 720p), "
+                         "batch (64 frames), "
                          "diff (adjoint), interactive (repeated 1-spp launches), two_level (flat vs two-level "
                          "layouts)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched leg")
@@ -919,7 +920,7 @@ def main():
     if args.no_diff:
         legs.discard("diff")
     fr, raw = frame_leg(args, level, w, h, spp, depth) if "frame" in legs else (None, None)
-    c2 = c2_leg() if "c2" in legs and args.workload == "c3" else None
+    l4 = level4_leg() if "level4" in legs and args.workload == "level6-1080p" else None
     batched = None
     if "batch" in legs:
         batched = batched_leg(args.batch_reps, args.retune, log)
@@ -928,7 +929,7 @@ def main():
     inter = interactive_leg() if "interactive" in legs else None
     two = two_level_leg() if "two_level" in legs else None
 
-    cpu = cpu_ref = c1 = rmse = None
+    cpu = cpu_ref = rtow_small = rmse = None
     threads = max(1, min(16, os.cpu_count() or 1))
     if fr is not None and not (args.no_cpu and args.no_rmse):
         of = OracleFrame(level, w, h)
@@ -942,7 +943,7 @@ def main():
         cpu_ref = cpu_reference(args.cpu_seconds)
         if cpu_ref:
             log("reference cpu tracer: %.2f Mrays/s on 1 thread" % cpu_ref["value"])
-        c1 = cpu_c1(threads)
+        rtow_small = cpu_rtow_small(threads)
 
     fr = fr or {}
     out = {
@@ -956,7 +957,7 @@ def main():
         "samples_per_s": fr.get("samples_per_s"), "Msamples_per_s": fr.get("Msamples_per_s"),
         "rays_per_frame": fr.get("rays"), "rays_per_sample": fr.get("rays_per_sample"),
         "bvh": fr.get("bvh"), "roofline": fr.get("roofline"), "rmse": rmse,
-        "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_c1": c1, "c2": c2,
+        "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_rtow_400x225": rtow_small, "level4_720p": l4,
         "batched_speedup": batched["speedup"] if batched else None, "batched": batched,
         "bwd_fwd_ratio": diff["ratio"] if diff else None, "differentiable": diff,
         "interactive": inter, "two_level": two,

@@ -273,7 +273,8 @@ rtx_status rtx_launch(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* st
  * with its own kernels): calls that overlap on one GPU race. */
 rtx_status rtx_launch_noalloc(rtx_ctx* ctx, const rtx_lp_general* lp, int once, void* stream);
 
-/* Batched render (no reference counterpart; SURVEY config 4): n frames, each
+/* Batched render (no reference counterpart; for callers that render many
+ * small frames, e.g. 64 uneven scenes at once): n frames, each
  * with its own launch parameters (scene handle, camera, size, spp, depth,
  * output buffers), rendered by ONE persistent kernel launch that pulls
  * 64-pixel tiles of all frames from one work counter, largest frames first.
@@ -289,7 +290,7 @@ rtx_status rtx_batch_create(rtx_ctx* ctx, const rtx_lp_general* lps, uint32_t n,
 rtx_status rtx_batch_launch(rtx_batch* batch, void* stream);
 rtx_status rtx_batch_destroy(rtx_batch* batch);
 
-/* ---- differentiable rendering (no reference counterpart; SURVEY config 5) --
+/* ---- differentiable rendering (no reference counterpart) ------------------
  * Material parameters are float4 per instance (instance = thing id):
  * (albedo r, g, b, fuzz).  rtx_scene_set_materials writes them into the
  * built scene on the device (diffuse instances take the albedo, metal

@@ -55,9 +55,9 @@ def test_tuning_json_roundtrip(tmp_path):
     assert autotune.load(p, "gfx950", "batch64") is None
     tun = dict(tile_w=16, stack_lds=32, blocks_per_cu=4, shade_batch=12, inner_frac=36)
     autotune.save(p, "gfx950", "batch64", tun, 0.5, note="x")
-    autotune.save(p, "gfx950", "c3", dict(tun, tile_w=8), 1.0)
+    autotune.save(p, "gfx950", "frame", dict(tun, tile_w=8), 1.0)
     assert autotune.load(p, "gfx950", "batch64") == tun
-    assert autotune.load(p, "gfx950", "c3")["tile_w"] == 8
-    assert autotune.load(p, "gfx942", "c3") is None
+    assert autotune.load(p, "gfx950", "frame")["tile_w"] == 8
+    assert autotune.load(p, "gfx942", "frame") is None
     with open(p) as f:
         assert json.load(f)["gfx950"]["batch64"]["seconds"] == 0.5

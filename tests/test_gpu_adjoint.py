@@ -69,7 +69,7 @@ class Frame:
 
 
 @pytest.fixture(scope="module")
-def c5(rtx):
+def dframe(rtx):
     say("level-4 scene, 640x360, 64 spp, depth 8")
     f = Frame(rtx, 4, 640, 360, 64, 8)
     f.mt = _target_params(f.m0, f.types)
@@ -90,35 +90,35 @@ def _adjoint(f, I, T):
     return f.la.adjoint((2.0 * (I.astype(np.float64) - T)).astype(np.float32)).astype(np.float64)
 
 
-def test_materials_roundtrip(c5, rtx):
+def test_materials_roundtrip(dframe, rtx):
     from rtxplay_amd import TYPE_DIFFUSE, TYPE_REFLECT
-    spec = c5.sc.things_spec
+    spec = dframe.sc.things_spec
     for k, t in enumerate(spec):
         if t["type"] in (TYPE_DIFFUSE, TYPE_REFLECT):
-            assert np.array_equal(c5.m0[k, :3], np.array(t["albedo"], np.float32)), k
+            assert np.array_equal(dframe.m0[k, :3], np.array(t["albedo"], np.float32)), k
         if t["type"] == TYPE_REFLECT:
-            assert c5.m0[k, 3] == np.float32(t["fuzz"])
-    c5.sc.set_materials(c5.mt)
-    got = c5.sc.get_materials()
-    c5.sc.set_materials(c5.m0)
-    for k, t in enumerate(c5.types):
+            assert dframe.m0[k, 3] == np.float32(t["fuzz"])
+    dframe.sc.set_materials(dframe.mt)
+    got = dframe.sc.get_materials()
+    dframe.sc.set_materials(dframe.m0)
+    for k, t in enumerate(dframe.types):
         if t == TYPE_DIFFUSE:
-            assert np.array_equal(got[k, :3], c5.mt[k, :3])
+            assert np.array_equal(got[k, :3], dframe.mt[k, :3])
         elif t == TYPE_REFLECT:
-            assert np.array_equal(got[k], c5.mt[k])
+            assert np.array_equal(got[k], dframe.mt[k])
         else:
-            assert np.array_equal(got[k], c5.m0[k])          # dielectric: untouched
+            assert np.array_equal(got[k], dframe.m0[k])          # dielectric: untouched
 
 
-def test_set_materials_equals_a_scene_built_with_them(c5, rtx):
+def test_set_materials_equals_a_scene_built_with_them(dframe, rtx):
     from rtxplay_amd.scene import RTOWScene
     from rtxplay_amd import TYPE_DIFFUSE, TYPE_REFLECT
-    spec = [dict(t) for t in c5.sc.things_spec]
+    spec = [dict(t) for t in dframe.sc.things_spec]
     for k, t in enumerate(spec):
         if t["type"] in (TYPE_DIFFUSE, TYPE_REFLECT):
-            t["albedo"] = tuple(np.float32(x) for x in c5.mt[k, :3])
+            t["albedo"] = tuple(np.float32(x) for x in dframe.mt[k, :3])
         if t["type"] == TYPE_REFLECT:
-            t["fuzz"] = np.float32(c5.mt[k, 3])
+            t["fuzz"] = np.float32(dframe.mt[k, 3])
     s2 = RTOWScene(level=4, spec=spec)
     s2.load()
     lp2 = _lp(rtx, s2.build(), 160, 90, 8, 8)
@@ -129,38 +129,38 @@ def test_set_materials_equals_a_scene_built_with_them(c5, rtx):
     s2.close()
     f = Frame(rtx, 4, 160, 90, 8, 8)
     try:
-        got = f.render(c5.mt)
+        got = f.render(dframe.mt)
     finally:
         f.close()
     assert np.array_equal(got.view(np.uint32), fresh.view(np.uint32))
 
 
-def test_albedo_gradient_matches_finite_differences(c5, rtx):
-    """Full C5 frame: the 16 albedo channels with the largest gradients, the
+def test_albedo_gradient_matches_finite_differences(dframe, rtx):
+    """The full 640x360 level-4 frame: the 16 albedo channels with the largest gradients, the
     ground among them, against central differences of the GPU render."""
-    g = _adjoint(c5, c5.I, c5.T)
+    g = _adjoint(dframe, dframe.I, dframe.T)
     from rtxplay_amd import TYPE_REFRACT
-    assert np.all(g[[k for k, t in enumerate(c5.types) if t == TYPE_REFRACT]] == 0)
+    assert np.all(g[[k for k, t in enumerate(dframe.types) if t == TYPE_REFRACT]] == 0)
     cand = [(abs(g[k, c]), k, c) for k in range(len(g)) for c in range(3)]
     cand.sort(reverse=True)
     picks = [(k, c) for _, k, c in cand[:16]]
     assert any(k == 0 for k, _ in picks)
     worst = 0.0
     for k, c in picks:
-        mp, mm = c5.m0.copy(), c5.m0.copy()
-        mp[k, c] = np.float32(c5.m0[k, c] + H_FD)
-        mm[k, c] = np.float32(c5.m0[k, c] - H_FD)
-        fd = (_loss(c5.render(mp), c5.T) - _loss(c5.render(mm), c5.T)) / (float(mp[k, c]) - float(mm[k, c]))
+        mp, mm = dframe.m0.copy(), dframe.m0.copy()
+        mp[k, c] = np.float32(dframe.m0[k, c] + H_FD)
+        mm[k, c] = np.float32(dframe.m0[k, c] - H_FD)
+        fd = (_loss(dframe.render(mp), dframe.T) - _loss(dframe.render(mm), dframe.T)) / (float(mp[k, c]) - float(mm[k, c]))
         rel = abs(g[k, c] - fd) / abs(fd)
         worst = max(worst, rel)
         say("albedo", k, c, "adjoint %.6e fd %.6e rel %.2e" % (g[k, c], fd, rel))
         assert rel <= REL, (k, c, g[k, c], fd)
-    c5.sc.set_materials(c5.m0)
+    dframe.sc.set_materials(dframe.m0)
     say("worst albedo rel err %.2e" % worst)
 
 
-def test_fuzz_gradient_matches_frozen_differences_on_the_c5_frame(c5, rtx, orc):
-    """The full config-5 frame: the 3 metals with the largest fuzz
+def test_fuzz_gradient_matches_frozen_differences_on_the_full_frame(dframe, rtx, orc):
+    """The full 640x360 level-4 frame: the 3 metals with the largest fuzz
     gradients against central differences (h = 1e-3) of the oracle's
     frozen-structure render, the interior derivative the adjoint computes
     (the jumps of paths that cross a facet edge, a silhouette or the
@@ -169,29 +169,29 @@ def test_fuzz_gradient_matches_frozen_differences_on_the_c5_frame(c5, rtx, orc):
     pixels where the GPU image and the oracle's agree."""
     from rtxplay_amd import TYPE_REFLECT
     meshes = {4: orc.sphere_mesh(1.0, 4)}
-    tris, inst = orc.flatten(meshes, c5.sc.things_spec, [4] * len(c5.types))
-    sc = orc.Scene(tris, inst, c5.sc.things_spec)
-    cam = orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, c5.w / c5.h, 0.1, 10.0)
-    say("oracle frozen render of the C5 frame")
-    agree = np.all(np.abs(sc.render_frozen(cam, c5.w, c5.h, c5.spp, c5.depth, c5.m0) - c5.I) <= 1e-5, axis=2)
+    tris, inst = orc.flatten(meshes, dframe.sc.things_spec, [4] * len(dframe.types))
+    sc = orc.Scene(tris, inst, dframe.sc.things_spec)
+    cam = orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, dframe.w / dframe.h, 0.1, 10.0)
+    say("oracle frozen render of the 640x360 frame")
+    agree = np.all(np.abs(sc.render_frozen(cam, dframe.w, dframe.h, dframe.spp, dframe.depth, dframe.m0) - dframe.I) <= 1e-5, axis=2)
     say("pixels where GPU and oracle agree: %.4f" % agree.mean())
     assert agree.mean() > 0.99
     M = agree[..., None].astype(np.float64)
-    g = _adjoint(c5, c5.I, c5.T + (1.0 - M) * (c5.I - c5.T))
+    g = _adjoint(dframe, dframe.I, dframe.T + (1.0 - M) * (dframe.I - dframe.T))
     # central differences need fuzz >= h (the big metal sphere's fuzz is 0)
-    metals = sorted([k for k, t in enumerate(c5.types) if t == TYPE_REFLECT and c5.m0[k, 3] >= 2 * H_FD],
+    metals = sorted([k for k, t in enumerate(dframe.types) if t == TYPE_REFLECT and dframe.m0[k, 3] >= 2 * H_FD],
                     key=lambda k: -abs(g[k, 3]))
 
     def loss(m):
-        return float((M * (sc.render_frozen(cam, c5.w, c5.h, c5.spp, c5.depth, m) - c5.T) ** 2).sum())
+        return float((M * (sc.render_frozen(cam, dframe.w, dframe.h, dframe.spp, dframe.depth, m) - dframe.T) ** 2).sum())
 
     for k in metals[:3]:
-        mp, mm = c5.m0.copy(), c5.m0.copy()
-        mp[k, 3] = np.float32(c5.m0[k, 3] + H_FD)
-        mm[k, 3] = np.float32(c5.m0[k, 3] - H_FD)
+        mp, mm = dframe.m0.copy(), dframe.m0.copy()
+        mp[k, 3] = np.float32(dframe.m0[k, 3] + H_FD)
+        mm[k, 3] = np.float32(dframe.m0[k, 3] - H_FD)
         fd = (loss(mp) - loss(mm)) / (float(mp[k, 3]) - float(mm[k, 3]))
         rel = abs(g[k, 3] - fd) / max(abs(fd), 1e-12)
-        say("C5 fuzz", k, "adjoint %.6e frozen fd %.6e rel %.2e" % (g[k, 3], fd, rel))
+        say("fuzz", k, "adjoint %.6e frozen fd %.6e rel %.2e" % (g[k, 3], fd, rel))
         assert rel <= REL, (k, g[k, 3], fd)
 
 

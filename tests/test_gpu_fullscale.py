@@ -25,7 +25,7 @@ ROWS = [int(y) for y in np.linspace(8, H - 9, 12)]
 
 
 @pytest.fixture(scope="module")
-def c3(rtx, orc):
+def level6(rtx, orc):
     from rtxplay_amd.scene import RTOWScene
     say("build level-6 scene on the GPU")
     s = RTOWScene(seed=0, half=11, level=6)
@@ -40,8 +40,8 @@ def c3(rtx, orc):
 
 
 @pytest.fixture(scope="module")
-def frame(rtx, c3):
-    s = c3[0]
+def frame(rtx, level6):
+    s = level6[0]
     lp = rtx.LpGeneral()
     lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = W, H, SPP, DEPTH, s.handle
     lp.camera = rtx.Camera().set((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, W / H, 0.1, 10.0).c
@@ -53,28 +53,28 @@ def frame(rtx, c3):
     return out
 
 
-def test_c3_flatten_bit_exact(c3):
-    s, tris, inst, _ = c3
+def test_level6_flatten_bit_exact(level6):
+    s, tris, inst, _ = level6
     g, gi = s.triangles()
     assert g.shape[0] == len(tris) == 7946240
     assert np.array_equal(g.reshape(-1, 9).view(np.uint32), tris.view(np.uint32))
     assert np.array_equal(gi, inst)
 
 
-def test_c3_tree_is_unchanged(c3):
-    """The builder's config-3 tree, pinned like the level-4 one: its build runs
+def test_level6_tree_is_unchanged(level6):
+    """The builder's tree of the level-6 scene, pinned like the level-4 one: its build runs
     the chunked (wide) levels, a block per task, a wave per task and, on the
     deepest levels, a thread per task of at most 16 primitives; every path
     makes the same decisions, so the tree is the one the builder gave before
     the thread-per-task path and the merged wide bins existed (recorded from
     it)."""
-    st = c3[0].stats()
+    st = level6[0].stats()
     assert (st["num_nodes"], st["num_leaves"], st["max_depth"]) == (4625294, 4625295, 31)
     assert abs(st["sah_cost"] - 37.87184524536133) < 1e-9
 
 
-def test_c3_hit_ids(orc, c3):
-    s, tris, inst, bvh = c3
+def test_level6_hit_ids(orc, level6):
+    s, tris, inst, bvh = level6
     cam = orc.camera((13, 2, 3), (0, 0, 0), (0, 1, 0), 20.0, W / H, 0.1, 10.0)
     rays = orc.primary_rays(cam, W, H)[::13].copy()         # 159,508 camera rays over the frame
     ref = bvh.closest_f64(rays, nthreads=THREADS)
@@ -91,9 +91,9 @@ def test_c3_hit_ids(orc, c3):
     _check_hits(s, tris, inst, b, bvh.closest_f64(b, nthreads=THREADS), want_ratio=0.9)
 
 
-def test_c3_rows_match_oracle(orc, c3, frame):
+def test_level6_rows_match_oracle(orc, level6, frame):
     say("oracle: %d rows at %d spp" % (len(ROWS), SPP))
-    raw, rpp, sq = _oracle_rows_parallel(orc, c3, SPP, sumsq=True)
+    raw, rpp, sq = _oracle_rows_parallel(orc, level6, SPP, sumsq=True)
     g = frame["raw"][ROWS]
     var = np.maximum(sq / SPP - raw ** 2, 0) / SPP
     sigma = np.sqrt(var.reshape(-1, 3).sum(axis=0)) / raw.reshape(-1, 3).shape[0]
@@ -108,15 +108,15 @@ def test_c3_rows_match_oracle(orc, c3, frame):
     assert (frame["rpp"][ROWS] == rpp).mean() >= 0.999
 
 
-def test_c3_rmse_vs_4096spp(orc, c3, frame):
+def test_level6_rmse_vs_4096spp(orc, level6, frame):
     """RMSE of the GPU's 500-spp rows against a 4096-spp CPU reference (its
     own streams), beside an independent CPU render at 500 spp (a third set
     of streams) and the RMSE that the reference's per-pixel sample variance
     predicts for unbiased estimates, sqrt(mean var x (1/500 + 1/4096))."""
     rows = [int(y) for y in np.linspace(40, H - 41, 8)]
     say("oracle: %d rows at 4096 spp (independent streams)" % len(rows))
-    ref, _, sq = _oracle_rows_parallel(orc, c3, 4096, seed_base=W * H, rows=rows, sumsq=True)
-    cpu, _, _ = _oracle_rows_parallel(orc, c3, SPP, seed_base=2 * W * H, rows=rows)
+    ref, _, sq = _oracle_rows_parallel(orc, level6, 4096, seed_base=W * H, rows=rows, sumsq=True)
+    cpu, _, _ = _oracle_rows_parallel(orc, level6, SPP, seed_base=2 * W * H, rows=rows)
     gpu = frame["raw"][rows]
     var = np.maximum(sq.astype(np.float64) / 4096 - ref.astype(np.float64) ** 2, 0)
     pred = float(np.sqrt(var.mean() * (1.0 / SPP + 1.0 / 4096)))
@@ -127,12 +127,12 @@ def test_c3_rmse_vs_4096spp(orc, c3, frame):
     assert abs(rmse_gpu - pred) <= 0.10 * pred and abs(rmse_cpu - pred) <= 0.10 * pred
 
 
-def _oracle_rows_parallel(orc, c3, spp, seed_base=0, sumsq=False, rows=None):
+def _oracle_rows_parallel(orc, level6, spp, seed_base=0, sumsq=False, rows=None):
     """Renders the listed rows with the oracle, THREADS rows at a time."""
     import ctypes
     from concurrent.futures import ThreadPoolExecutor
     rows = ROWS if rows is None else rows
-    s, tris, inst, bvh = c3
+    s, tris, inst, bvh = level6
     sc = orc.Scene.__new__(orc.Scene)
     sc.tris, sc.tri_inst, sc.bvh = bvh.tris, np.ascontiguousarray(inst, np.uint32), bvh
     sc.things = (orc.Thing * len(s.things_spec))()
