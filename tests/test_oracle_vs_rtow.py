@@ -20,7 +20,7 @@ to three known differences, each accounted for:
     the tessellation level; the tests measure it as the change from a coarser
     tessellation and allow it on top of the Monte Carlo error.
 
-Checks at two depths (rtow depth 15 = lp.depth 16, the configs' depth; and
+Checks at two depths (rtow depth 15 = lp.depth 16, the benchmark frames' depth; and
 rtow depth 2 = lp.depth 3, where the depth limit decides most paths): the
 per-channel image means agree within 3 sigma plus the tessellation term,
 the 25 x 25-pixel tile means agree (chi^2 per tile near 1, no tile beyond
