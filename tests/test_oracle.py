@@ -205,6 +205,43 @@ def test_scene_recipe_shape(orc):
     assert (np.linalg.norm(c - np.array([4, .2, 0]), axis=1) > 0.9).all()
 
 
+def test_scene_recipe_draw_ranges(orc):
+    """The draws of optx/rtwo.cxx:165-207: a small sphere of radius .2 at
+    (a + .9u, .2, b + .9u) for each grid cell a, b in [-11, 11) at most once;
+    diffuse albedo = rnd3 * rnd3 in [0, 1), metal albedo in [.5, 1) and fuzz in
+    [0, .5), glass index 1.5; the ground and the three big spheres as listed
+    (optx/rtwo.cxx:154-163, 209-243)."""
+    for seed in (0, 5, 17):
+        s = orc.scene_rtow(seed, 11)
+        g = s[0]
+        assert (g.radius, tuple(g.center), g.type) == (1000, (0, -1000, 0), orc.ORC_DIFFUSE)
+        assert np.array_equal(np.array(list(g.albedo), np.float32), np.float32([.5, .5, .5]))
+        big = s[-3:]
+        assert [(b.type, tuple(b.center), b.radius) for b in big] == [
+            (orc.ORC_REFRACT, (0, 1, 0), 1), (orc.ORC_DIFFUSE, (-4, 1, 0), 1), (orc.ORC_REFLECT, (4, 1, 0), 1)]
+        assert np.float32(big[0].index) == np.float32(1.5)
+        assert np.array_equal(np.array(list(big[1].albedo), np.float32), np.float32([.4, .2, .1]))
+        assert np.array_equal(np.array(list(big[2].albedo), np.float32), np.float32([.7, .6, .5]))
+        assert big[2].fuzz == 0
+        cells = set()
+        for t in s[1:-3]:
+            x, y, z = (float(v) for v in t.center)
+            assert np.float32(t.radius) == np.float32(.2) and np.float32(y) == np.float32(.2)
+            a, b = int(np.floor(x)), int(np.floor(z))
+            assert -11 <= a < 11 and -11 <= b < 11 and (a, b) not in cells
+            cells.add((a, b))
+            assert 0 <= x - a <= .9 + 1e-5 and 0 <= z - b <= .9 + 1e-5
+            alb = np.array(list(t.albedo), np.float64)
+            if t.type == orc.ORC_DIFFUSE:
+                assert (alb >= 0).all() and (alb < 1).all()
+            elif t.type == orc.ORC_REFLECT:
+                assert (alb >= .5).all() and (alb < 1).all() and 0 <= t.fuzz < .5
+            else:
+                assert t.type == orc.ORC_REFRACT and np.float32(t.index) == np.float32(1.5)
+        # the rejection around (4, .2, 0) skips at most the few cells near it
+        assert 480 <= len(cells) + 4 <= 488
+
+
 # ---- closest hit --------------------------------------------------------------
 def _ray(o, d, tmin=1e-3, tmax=1e16):
     return np.array(list(o) + [tmin] + list(d) + [tmax], np.float32)
