@@ -40,6 +40,12 @@
  *                        within 3 sigma plus the measured tessellation term,
  *                        25x25 tiles within chi^2 < 1.6 per tile and 5 sigma,
  *                        rays per sample.
+ *   orc_scene_rtow       the draw ranges and fixed spheres of optx/rtwo.cxx:
+ *                        154-243 (one small sphere of radius .2 per grid cell
+ *                        at most, offsets below .9, diffuse albedo in [0,1),
+ *                        metal albedo in [.5,1) and fuzz in [0,.5), glass
+ *                        1.5), the material mix, the exclusion around
+ *                        (4,.2,0); the product's host recipe equals it.
  *   orc_pp_srgb          IEC 61966-2-1 curve evaluated in FP64.
  *   orc_render_frozen    equals orc_render at the scene's own materials (to FP32
  *                        rounding); its albedo dependence is a product (checked
