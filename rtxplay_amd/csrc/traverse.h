@@ -344,8 +344,10 @@ __device__ __forceinline__ void wave_sync() {
 // owner (the first lane whose prefix exceeds p) by a binary search of the
 // prefix array in LDS and reads the owner's ray and leaves with
 // ds_bpermute.  A hit goes into the owner's LDS slot as the 64-bit minimum
-// of (t, global primitive id) -- t > tmin >= 0, so the float's bits order
-// like its value -- and the lane whose key won writes its lane number next
+// of (t, primitive id) -- t > tmin >= 0, so the float's bits order like its
+// value; the id is global in a flat scene and within the mesh in a
+// two-level one, where all of a lane's parked leaves belong to the instance
+// it is in -- and the lane whose key won writes its lane number next
 // to it; the owner then takes that hit (t, u, v, triangle) under the rule of
 // the per-lane loop: closer, or as close with the lower id.  The minimum of
 // (t, id) does not depend on the order of the tests, so the closest hit is
