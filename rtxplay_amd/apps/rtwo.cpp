@@ -3,7 +3,7 @@
 // C-ABI (through include/rtx/rtx.hpp).
 //
 //   rtwo [-g WxH|NAME] [-a W:H] [-s spp] [-d depth] [-A RPP] [-G] [-D TYPE] [-S] [-q] [-v] [-t] [-h]
-//        [-l level|mixed] [-r seed] [-I] > out.ppm
+//        [-l level|mixed] [-r seed|glibc] [-I] > out.ppm
 //
 // The reference's batch options (optx/args.cxx:23-121) with their long
 // forms: -g/--geometry takes WxH or a named resolution (optx/args.h:20-42),
@@ -15,7 +15,8 @@
 // of this build: the image stays as rendered, and -G prints the guides
 // (normals, albedos) only with -D, as the reference does.
 // Options of this driver only: -l the subdivision level of every sphere
-// (default: the reference's mixed levels), -r the scene seed, -I the
+// (default: the reference's mixed levels), -r the scene seed (glibc: the
+// reference's own rand() stream, the scene its batch binary renders), -I the
 // reference's two-level layout (an instance structure over one structure
 // per mesh) instead of the flat one.
 //
@@ -53,11 +54,23 @@ struct Frand48 {
     float range(float lo, float hi) { return lo + (*this)() * (hi - lo); }
 };
 
-// RTWO::load (optx/rtwo.cxx:133-245) drawn from one seeded Frand48 stream:
-// per cell select, center x, center z, then the material draws.
+// util::rnd() of the reference's host code (optx/util_cpu.h:90-91): the C
+// library's rand() over (RAND_MAX + 1), from srand(1) -- the stream
+// RTWO::load draws from, since it never seeds
+struct GlibcRand {
+    GlibcRand() { std::srand(1); }
+    float operator()() { return (float)std::rand() / ((float)RAND_MAX + 1.f); }
+    float range(float lo, float hi) { return lo + (*this)() * (hi - lo); }
+};
+
+// RTWO::load (optx/rtwo.cxx:133-245) drawn from one seeded Frand48 stream
+// (per cell select, center x, center z, then the material draws, triples x
+// first), or with -r glibc from the reference's own stream, whose V::rnd()
+// triples g++ draws z first (it evaluates make_float3's arguments right to
+// left): then the scene is the one the reference's batch binary renders.
 class RTWO : public rtx::Scene {
    public:
-    RTWO(uint32_t seed, int level) : seed_(seed), level_(level) {}
+    RTWO(uint32_t seed, int level, bool glibc = false) : seed_(seed), level_(level), glibc_(glibc) {}
 
     unsigned int load() override {
         // level_ < 0: the reference's mixed levels (optx/rtwo.cxx:138-145)
@@ -69,8 +82,31 @@ class RTWO : public rtx::Scene {
                 gas[l] = add(s.mesh());
             }
         }
-        auto pick = [&](int mixed) { return gas[level_ >= 0 ? level_ : mixed]; };
+        if (glibc_) {
+            GlibcRand rnd;
+            return recipe(gas, rnd, true);
+        }
         Frand48 rnd(seed_);
+        return recipe(gas, rnd, false);
+    }
+
+   private:
+    // one triple of draws (V::rnd, optx/v.h:80-81) in the stream's order
+    template <class R>
+    static void triple(R& rnd, bool zfirst, float lo, float hi, float& x, float& y, float& z) {
+        if (zfirst) {
+            z = rnd.range(lo, hi);
+            y = rnd.range(lo, hi);
+            x = rnd.range(lo, hi);
+        } else {
+            x = rnd.range(lo, hi);
+            y = rnd.range(lo, hi);
+            z = rnd.range(lo, hi);
+        }
+    }
+    template <class R>
+    unsigned int recipe(const unsigned int (&gas)[10], R& rnd, bool zfirst) {
+        auto pick = [&](int mixed) { return gas[level_ >= 0 ? level_ : mixed]; };
         rtx::Thing thing{};
         float xf[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         auto place = [&](unsigned id, float r, float cx, float cy, float cz) {
@@ -89,13 +125,15 @@ class RTWO : public rtx::Scene {
                 const float dx = cx - 4.f, dy = .2f - .2f, dz = cz - 0.f;
                 if (std::sqrt(dx * dx + dy * dy + dz * dz) > .9f) {
                     if (select < .8f) {
-                        const float r1 = rnd(), g1 = rnd(), b1 = rnd();
-                        const float r2 = rnd(), g2 = rnd(), b2 = rnd();
+                        float r1, g1, b1, r2, g2, b2;
+                        triple(rnd, zfirst, 0.f, 1.f, r1, g1, b1);
+                        triple(rnd, zfirst, 0.f, 1.f, r2, g2, b2);
                         thing.optics.type = RTX_OPTICS_DIFFUSE;
                         thing.optics.diffuse.albedo = {r1 * r2, g1 * g2, b1 * b2};
                         place(add(thing, pick(6)), .2f, cx, .2f, cz);
                     } else if (select < .95f) {
-                        const float r1 = rnd.range(.5f, 1.f), g1 = rnd.range(.5f, 1.f), b1 = rnd.range(.5f, 1.f);
+                        float r1, g1, b1;
+                        triple(rnd, zfirst, .5f, 1.f, r1, g1, b1);
                         const float fz = rnd.range(0.f, .5f);
                         thing.optics.type = RTX_OPTICS_REFLECT;
                         thing.optics.reflect.albedo = {r1, g1, b1};
@@ -123,14 +161,14 @@ class RTWO : public rtx::Scene {
         return id + 1;
     }
 
-   private:
     uint32_t seed_;
     int level_;
+    bool glibc_;
 };
 
 void usage() {
     std::cerr << "usage: rtwo [-g WxH|NAME] [-a W:H] [-s spp] [-d depth] [-A RPP] [-G] [-D TYPE] [-S] [-q] [-v] [-t] [-h]\n"
-                 "            [-l level|mixed] [-r seed] [-I]\n";
+                 "            [-l level|mixed] [-r seed|glibc] [-I]\n";
 }
 
 // named resolutions of the -g option (optx/args.h:20-42)
@@ -163,6 +201,7 @@ int main(int argc, char* argv[]) {
     unsigned spp = 50, depth = 16;
     int level = -1;
     uint32_t seed = 0;
+    bool glibc = false;
     bool stats = false, quiet = false, aov_rpp = false, two_level = false, guides = false, denoise = false;
     int flag_v = 0, flag_t = 0, flag_q = 0, flag_h = 0, flag_G = 0, flag_S = 0;
     const struct option lopts[] = {
@@ -228,7 +267,10 @@ int main(int argc, char* argv[]) {
                 break;
             }
             case 'l': level = std::string(optarg) == "mixed" ? -1 : std::atoi(optarg); break;
-            case 'r': seed = (uint32_t)std::strtoul(optarg, nullptr, 10); break;
+            case 'r':
+                if (std::strcmp(optarg, "glibc") == 0) glibc = true;
+                else seed = (uint32_t)std::strtoul(optarg, nullptr, 10);
+                break;
             case 'I': two_level = true; break;
             default: usage(); return 1;   // optx/args.cxx:96-98
         }
@@ -250,7 +292,7 @@ int main(int argc, char* argv[]) {
         camera.set({13.f, 2.f, 3.f}, {0.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, 20.f, (float)w / (float)h, .1f, 10.f);
         lp.camera = camera.pod();
 
-        RTWO rtwo(seed, level);
+        RTWO rtwo(seed, level, glibc);
         if (two_level) rtwo.build_mode(RTX_BUILD_TWO_LEVEL);
         rtwo.load();
         rtwo.build(&lp.is_handle);

@@ -69,6 +69,71 @@ def sphere_things(seed=0, half=11):
     return out
 
 
+class GlibcRand:
+    """util::rnd() of the reference's host code (optx/util_cpu.h:90-91): the
+    C library's rand() over (RAND_MAX + 1), in FP32, from srand(1) -- the
+    stream RTWO::load draws from, since it never seeds.  (rand() keeps one
+    state per process; the recipe below resets it.)"""
+
+    def __init__(self):
+        import ctypes
+        self.libc = ctypes.CDLL("libc.so.6")
+        self.libc.srand(1)
+        self.scale = F(2147483647) + F(1)
+
+    def __call__(self):
+        return F(self.libc.rand()) / self.scale
+
+    def range(self, lo, hi):
+        return F(lo) + self() * (F(hi) - F(lo))
+
+    def v3(self, lo=None, hi=None):
+        """V::rnd() / V::rnd(min, max) (optx/v.h:80-81): make_float3 of three
+        draws, whose arguments g++ evaluates right to left (z drawn first),
+        as the byte-exact restatement of rtow.cxx confirms for the same
+        pattern there"""
+        d = (lambda: self()) if lo is None else (lambda: self.range(lo, hi))
+        z = d()
+        y = d()
+        x = d()
+        return x, y, z
+
+
+def sphere_things_glibc(half=11):
+    """RTWO::load (optx/rtwo.cxx:154-243) on the reference's own stream: the
+    same sphere list the reference's batch binary renders (at half 11: 392
+    diffuse, 63 metal and 27 glass small spheres plus the ground and three
+    big ones).  The brace-initialised center draws x before z; a diffuse
+    albedo is the component product of two V::rnd() triples."""
+    rng = GlibcRand()
+    out = [dict(type=TYPE_DIFFUSE, albedo=(F(.5), F(.5), F(.5)), fuzz=F(0), index=F(0),
+                center=(F(0), F(-1000), F(0)), radius=F(1000), kind=0)]
+    for a in range(-half, half):
+        for b in range(-half, half):
+            select = rng()
+            cx = F(a) + F(.9) * rng()
+            cz = F(b) + F(.9) * rng()
+            c = (cx, F(.2), cz)
+            dx, dy, dz = cx - F(4), F(.2) - F(.2), cz - F(0)
+            if np.sqrt(dx * dx + dy * dy + dz * dz) > F(.9):
+                if select < F(.8):
+                    p = rng.v3()
+                    q = rng.v3()
+                    alb = (p[0] * q[0], p[1] * q[1], p[2] * q[2])
+                    out.append(dict(type=TYPE_DIFFUSE, albedo=alb, fuzz=F(0), index=F(0), center=c, radius=F(.2),
+                                    kind=1))
+                elif select < F(.95):
+                    alb = rng.v3(.5, 1.)
+                    fz = rng.range(0., .5)
+                    out.append(dict(type=TYPE_REFLECT, albedo=alb, fuzz=fz, index=F(0), center=c, radius=F(.2),
+                                    kind=1))
+                else:
+                    out.append(dict(type=TYPE_REFRACT, albedo=(F(0), F(0), F(0)), fuzz=F(0), index=F(1.5),
+                                    center=c, radius=F(.2), kind=1))
+    out.extend(sphere_things(0, 0)[1:])     # the three big spheres
+    return out
+
+
 # the reference's own per-sphere tessellation levels (optx/rtwo.cxx:138-241)
 def mixed_level(thing, index, count):
     if thing["kind"] == 0:
@@ -88,9 +153,12 @@ class RTOWScene(Scene):
     """RTWO::load (optx/rtwo.cxx:133-245) over unit-sphere meshes.
 
     level: an int (every sphere at that subdivision level) or "mixed" (the
-    reference's per-sphere levels); spec: a sphere_things-style list to load
-    instead of the seeded one; two_level: build the reference's layout (one
-    structure per mesh under an instance structure) instead of the flat one."""
+    reference's per-sphere levels); seed: the Frand48 seed of the sphere
+    list, or "glibc" for the reference's own stream (sphere_things_glibc:
+    with level "mixed", the scene the reference's batch binary renders);
+    spec: a sphere_things-style list to load instead; two_level: build the
+    reference's layout (one structure per mesh under an instance structure)
+    instead of the flat one."""
 
     def __init__(self, seed=0, half=11, level=4, device=0, spec=None, two_level=False):
         super().__init__(device)
@@ -101,7 +169,12 @@ class RTOWScene(Scene):
         self._spec = spec
 
     def load(self):
-        spec = self._spec if self._spec is not None else sphere_things(self.seed, self.half)
+        if self._spec is not None:
+            spec = self._spec
+        elif self.seed == "glibc":
+            spec = sphere_things_glibc(self.half)
+        else:
+            spec = sphere_things(self.seed, self.half)
         levels = [self.level if not isinstance(self.level, str) else mixed_level(t, i, len(spec))
                   for i, t in enumerate(spec)]
         meshes = {}

@@ -392,19 +392,20 @@ def test_postprocess_matches_oracle(rtx, orc):
 
 
 # ---- the C++ batch driver over the same library ------------------------------------
-@pytest.mark.parametrize("two_level", [False, True])
-def test_cpp_rtwo_driver_matches_python_path(rtx, two_level):
+@pytest.mark.parametrize("two_level,recipe", [(False, 0), (True, 0), (False, "glibc")])
+def test_cpp_rtwo_driver_matches_python_path(rtx, two_level, recipe):
     """rtxplay_amd/rtwo (include/rtx/rtx.hpp over the C-ABI) renders the same
-    bytes as the Python mirror for the same seeded scene, flat and (-I)
-    two-level."""
+    bytes as the Python mirror for the same scene: seeded, flat and (-I)
+    two-level, and on the reference's own rand() stream (-r glibc)."""
     import subprocess
     from conftest import ROOT
     from rtxplay_amd.scene import RTOWScene
     exe = os.path.join(ROOT, "rtxplay_amd", "rtwo")
     w, h, spp, depth = 48, 27, 3, 16
     say("rtwo driver", w, h, spp, "two-level", two_level)
-    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", str(spp), "-d", str(depth), "-l", "2", "-A", "RPP", "-S"] +
-                       (["-I"] if two_level else []), capture_output=True, text=True, timeout=120)
+    p = subprocess.run([exe, "-g", "%dx%d" % (w, h), "-s", str(spp), "-d", str(depth), "-l", "2", "-A", "RPP", "-S",
+                        "-r", str(recipe)] + (["-I"] if two_level else []), capture_output=True, text=True,
+                       timeout=120)
     assert p.returncode == 0, p.stderr
     tok = p.stdout.split()
     assert tok[0] == "P3" and int(tok[1]) == w and int(tok[2]) == h
@@ -413,8 +414,8 @@ def test_cpp_rtwo_driver_matches_python_path(rtx, two_level):
     assert tok[k] == "P2"
     pgm = np.array(tok[k + 4:k + 4 + w * h], np.int64).reshape(h, w)[::-1]
     assert "(pixel, rays, milliseconds)" in p.stderr
-    s = RTOWScene(seed=0, half=11, level=2, two_level=two_level)
-    s.load()
+    s = RTOWScene(seed=recipe, half=11, level=2, two_level=two_level)
+    assert s.load() == (486 if recipe == "glibc" else len(s.things_spec))
     s.build()
     L, lp = _launch(rtx, s, w, h, spp, depth)
     out = rtx.DeviceBuffer(4 * w * h)
