@@ -1,7 +1,8 @@
 """Benchmark of the MI355X ray-tracing hot path.
 
-One step = one render of the RTOW final scene with every sphere tessellated
-at level 6 (7.95M world-space triangles, seed 0), 1920x1080, 500 spp, depth 16
+One step = one render of the RTOW final scene as the reference builds it
+(RTWO::load on its own rand() stream: 486 spheres), every sphere tessellated
+at level 6 (7.96M world-space triangles), 1920x1080, 500 spp, depth 16
 (the render of optx/camera_r.cu + optx/optics_r.cu under one launch), with the
 acceleration structure already built and resident in HBM.
 
@@ -59,6 +60,9 @@ VALU_PEAK_GINST = CUS * PEAK_GHZ
 # per ray for its own input/output (ray 32 in, hit 16 out)
 NODE_BYTES, TRI_BYTES, SHADE_BYTES, IO_BYTES = 112, 48, 80, 48
 PROFILES = os.path.join(ROOT, "profiles")
+# the sphere list of every leg but the batch: RTWO::load on the reference's
+# own rand() stream (optx/rtwo.cxx:137-244), the scene its batch binary renders
+SCENE = "glibc"
 FRAME_KERNEL = "k_render<256, false, 32, false,"   # the frame kernel (either top-node staging)
 COPY_KERNEL = "k_pp<false>"
 # counter passes of the headline frame (each within one pass's block limits)
@@ -108,9 +112,9 @@ class OracleFrame:
 
     def __init__(self, level, w, h):
         import oracle
-        from rtxplay_amd.scene import sphere_things
+        from rtxplay_amd.scene import sphere_things_glibc
         t0 = time.perf_counter()
-        things = sphere_things(0, 11)
+        things = sphere_things_glibc(11)
         meshes = {level: oracle.sphere_mesh(1.0, level)}
         tris, inst = oracle.flatten(meshes, things, [level] * len(things))
         self.scene = oracle.Scene(tris, inst, things)
@@ -499,7 +503,7 @@ def diff_leg(reps):
     from rtxplay_amd.scene import RTOWScene
 
     w, h, spp, depth = 640, 360, 64, 8
-    sc = RTOWScene(seed=0, half=11, level=4)
+    sc = RTOWScene(seed=SCENE, half=11, level=4)
     sc.load()
     handle = sc.build()
     lp = rtx.LpGeneral()
@@ -668,7 +672,7 @@ def interactive_leg(frames=120):
     from rtxplay_amd.scene import RTOWScene
 
     w, h = 1280, 720
-    sc = RTOWScene(seed=0, half=11, level=6)
+    sc = RTOWScene(seed=SCENE, half=11, level=6)
     sc.load()
     lp = rtx.LpGeneral()
     lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, 1, 16, sc.build()
@@ -723,10 +727,10 @@ def two_level_leg(spp=64, steps=3):
     from rtxplay_amd.scene import RTOWScene, transform_for
 
     w, h = 1920, 1080
-    rec = {"workload": "rtow-final-scene, 1920x1080, %d spp, depth 16, seed 0" % spp}
+    rec = {"workload": "rtow-final-scene (the reference's rand() stream), 1920x1080, %d spp, depth 16" % spp}
     for level in (6, "mixed"):
         for tl in (False, True):
-            sc = RTOWScene(seed=0, half=11, level=level, two_level=tl)
+            sc = RTOWScene(seed=SCENE, half=11, level=level, two_level=tl)
             sc.load()
             lp = rtx.LpGeneral()
             lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, 16, sc.build()
@@ -771,7 +775,7 @@ def frame_leg(args, level, w, h, spp, depth):
     from rtxplay_amd.scene import RTOWScene
 
     t0 = time.perf_counter()
-    scene = RTOWScene(seed=0, half=11, level=level)
+    scene = RTOWScene(seed=SCENE, half=11, level=level)
     scene.load()
     handle = scene.build()
     if args.shade_batch:
@@ -836,7 +840,7 @@ def level4_leg(steps=5):
     from rtxplay_amd.hip import Event, Stream
     from rtxplay_amd.scene import RTOWScene
     level, w, h, spp, depth = WORKLOADS["level4-720p"]
-    sc = RTOWScene(seed=0, half=11, level=level)
+    sc = RTOWScene(seed=SCENE, half=11, level=level)
     sc.load()
     lp = rtx.LpGeneral()
     lp.image_w, lp.image_h, lp.spp, lp.depth, lp.is_handle = w, h, spp, depth, sc.build()
@@ -855,7 +859,8 @@ def level4_leg(steps=5):
     rays = int(la.rays_per_pixel().sum(dtype=np.uint64))
     la.free()
     sc.close()
-    rec = {"workload": "rtow-final-scene level-4 spheres (%d triangles), %dx%d, %d spp, depth %d, seed 0"
+    rec = {"workload": "rtow-final-scene level-4 spheres (%d triangles, the reference's rand() stream), %dx%d, "
+                       "%d spp, depth %d"
                        % (st["num_prims"], w, h, spp, depth), "ms_per_frame": round(ms, 3),
            "Mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
            "Msamples_per_s": round(w * h * spp / (ms * 1e-3) / 1e6, 1), "rays_per_frame": rays,
@@ -950,7 +955,8 @@ def main():
         "metric": METRIC, "value": fr.get("mrays"), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": fr.get("ms"), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "rtow-final-scene level-%d spheres (%s triangles), %dx%d, %d spp, depth %d, seed 0"
+        "config": {"workload": "rtow-final-scene level-%d spheres (%s triangles, the reference's rand() stream), "
+                               "%dx%d, %d spp, depth %d"
                                % (level, fr.get("triangles"), w, h, spp, depth),
                    "image": [w, h], "spp": spp, "depth": depth, "triangles": fr.get("triangles"),
                    "parallelism": "single-gpu"},
