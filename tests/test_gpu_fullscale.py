@@ -1,5 +1,6 @@
-"""Parity at the benchmark's full size: the level-6 RTOW scene (7.95M
-triangles), 1920x1080, 500 spp, depth 16.
+"""Parity at the benchmark's full size: the level-6 RTOW scene on the
+reference's own sphere list (RTWO::load on glibc rand() from srand(1), 486
+spheres, 7,962,624 triangles), 1920x1080, 500 spp, depth 16.
 
 * world triangles of the device flatten == the oracle's, bit for bit;
 * primitive ids of camera rays and their mirror bounces == the oracle's FP64
@@ -22,13 +23,15 @@ pytestmark = pytest.mark.gpu
 W, H, SPP, DEPTH = 1920, 1080, 500, 16
 THREADS = max(1, min(16, os.cpu_count() or 1))
 ROWS = [int(y) for y in np.linspace(8, H - 9, 12)]
+# nodes, leaves, depth and SAH cost of the builder's tree of the reference's level-6 scene
+TREE_GLIBC = (4635044, 4635045, 31, 37.871849060058594)
 
 
 @pytest.fixture(scope="module")
 def level6(rtx, orc):
     from rtxplay_amd.scene import RTOWScene
-    say("build level-6 scene on the GPU")
-    s = RTOWScene(seed=0, half=11, level=6)
+    say("build level-6 scene (the reference's sphere list) on the GPU")
+    s = RTOWScene(seed="glibc", half=11, level=6)
     s.load()
     s.build()
     meshes = {6: orc.sphere_mesh(1.0, 6)}
@@ -56,21 +59,35 @@ def frame(rtx, level6):
 def test_level6_flatten_bit_exact(level6):
     s, tris, inst, _ = level6
     g, gi = s.triangles()
-    assert g.shape[0] == len(tris) == 7946240
+    assert g.shape[0] == len(tris) == 7962624
     assert np.array_equal(g.reshape(-1, 9).view(np.uint32), tris.view(np.uint32))
     assert np.array_equal(gi, inst)
 
 
-def test_level6_tree_is_unchanged(level6):
-    """The builder's tree of the level-6 scene, pinned like the level-4 one: its build runs
-    the chunked (wide) levels, a block per task, a wave per task and, on the
-    deepest levels, a thread per task of at most 16 primitives; every path
-    makes the same decisions, so the tree is the one the builder gave before
-    the thread-per-task path and the merged wide bins existed (recorded from
-    it)."""
-    st = level6[0].stats()
+def test_level6_tree_is_unchanged(rtx):
+    """The builder's tree of the level-6 scene on the Frand48 sphere list of
+    seed 0 (485 spheres, 7,946,240 triangles), pinned like the level-4 one:
+    its build runs the chunked (wide) levels, a block per task, a wave per
+    task and, on the deepest levels, a thread per task of at most 16
+    primitives; every path makes the same decisions, so the tree is the one
+    the builder gave before the thread-per-task path and the merged wide bins
+    existed (recorded from it)."""
+    from rtxplay_amd.scene import RTOWScene
+    s = RTOWScene(seed=0, half=11, level=6)
+    s.load()
+    s.build()
+    st = s.stats()
+    s.close()
     assert (st["num_nodes"], st["num_leaves"], st["max_depth"]) == (4625294, 4625295, 31)
     assert abs(st["sah_cost"] - 37.87184524536133) < 1e-9
+
+
+def test_level6_reference_scene_tree_is_unchanged(level6):
+    """The builder's tree of the reference's level-6 scene (the benchmark's),
+    recorded from the builder whose seed-0 tree is pinned above."""
+    st = level6[0].stats()
+    assert (st["num_nodes"], st["num_leaves"], st["max_depth"]) == TREE_GLIBC[:3]
+    assert abs(st["sah_cost"] - TREE_GLIBC[3]) < 1e-9
 
 
 def test_level6_hit_ids(orc, level6):
