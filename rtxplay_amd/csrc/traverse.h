@@ -322,9 +322,9 @@ __device__ __forceinline__ bool first_active_lane() {
     return (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)e) - 1);
 }
 
-// LDS words per wave that the compacted leaf phase uses: the inclusive
-// prefix of the pair counts (64), the best key per lane (64 x 64 bit) and
-// the lane holding it (64)
+// LDS words per wave that the compacted leaf phase uses: the owner marks of
+// a round (64), the best key per lane (64 x 64 bit) and the lane holding it
+// (64)
 constexpr uint32_t kLeafScratchWords = 256;
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 
@@ -341,9 +341,9 @@ __device__ __forceinline__ void wave_sync() {
 // leaves hold c = c1 + c2 triangles; the wave's (lane, triangle) pairs are
 // numbered by an inclusive prefix sum of c and dealt out 64 at a time, one
 // per lane whatever lane parked them.  The lane that tests pair p finds its
-// owner (the first lane whose prefix exceeds p) by a binary search of the
-// prefix array in LDS and reads the owner's ray and leaves with
-// ds_bpermute.  A hit goes into the owner's LDS slot as the 64-bit minimum
+// owner (the lane whose pairs hold p) by a max-scan over the wave of the
+// round positions that the owners mark in LDS, and reads the owner's ray and
+// leaves with ds_bpermute.  A hit goes into the owner's LDS slot as the 64-bit minimum
 // of (t, primitive id) -- t > tmin >= 0, so the float's bits order like its
 // value; the id is global in a flat scene and within the mesh in a
 // two-level one, where all of a lane's parked leaves belong to the instance
@@ -357,6 +357,14 @@ __device__ __forceinline__ void wave_sync() {
 // if it lies in its owner's own leaves (q < the owner's count), and read
 // from the owner's triangle array, so every triangle read is one the owner
 // parked.
+// max(v, v of the lane that the DPP control CTRL names); lanes outside the
+// rows of ROW_MASK, or without a source lane in their row, keep v
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t max_dpp(uint32_t v) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+    return v > t ? v : t;
+}
+
 template <bool G, bool TL, bool COUNT>
 __device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint32_t f1, uint32_t c1, uint32_t f2,
                                                uint32_t c2, lds_u32* cl, uint32_t* wi) {
@@ -371,10 +379,9 @@ __device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     if (total == 0) return;
     const uint32_t excl = incl - c;
-    lds_u32* B = cl;
+    lds_u32* M = cl;
     lds_u64* K = (lds_u64*)(cl + 64);
     lds_u32* Wn = cl + 192;
-    B[lane] = incl;
     K[lane] = ~0ull;
     // what a helper lane needs of the owner: its ray (object space inside an
     // instance), the first pair's number, and its leaves
@@ -384,11 +391,23 @@ __device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint
         if (COUNT && lane == 0) wi[1]++;
         const uint32_t p = base + lane;
         const bool has = p < total;
-        // owner: the number of the first 63 prefixes that are <= p (B[63] = total > p)
-        uint32_t i = 0;
-#pragma unroll
-        for (uint32_t st = 32; st >= 1; st >>= 1)
-            if (B[i + st - 1u] <= p) i += st;
+        // owner: the lane whose pairs [excl, incl) hold p.  Each lane whose
+        // pairs meet this round marks the round position of its first one
+        // with its lane number; the owner of position j is then the largest
+        // mark at or before j (lane numbers grow with the pair numbers), an
+        // inclusive max-scan over the wave (DPP: shifts within the 16-lane
+        // rows, then broadcasts of the row ends to the rows above)
+        M[lane] = 0u;
+        wave_sync();
+        if (c != 0u && incl > base && excl < base + 64u) M[(excl > base ? excl : base) - base] = lane;
+        wave_sync();
+        uint32_t i = M[lane];
+        i = max_dpp<0x111, 0xf>(i);   // row_shr:1
+        i = max_dpp<0x112, 0xf>(i);   // row_shr:2
+        i = max_dpp<0x114, 0xf>(i);   // row_shr:4
+        i = max_dpp<0x118, 0xf>(i);   // row_shr:8
+        i = max_dpp<0x142, 0xa>(i);   // row_bcast:15 into rows 1 and 3
+        i = max_dpp<0x143, 0xc>(i);   // row_bcast:31 into rows 2 and 3
         const int ad = (int)(i << 2);
         auto bp = [&](uint32_t v) -> uint32_t { return (uint32_t)__builtin_amdgcn_ds_bpermute(ad, (int)v); };
         auto bpf = [&](float v) -> float { return __uint_as_float(bp(__float_as_uint(v))); };
@@ -486,7 +505,8 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
     if (going && s.node == kEmptyRoot) going = false;
     for (;;) {
         // node phase (no lane starts or stops going in it)
-        const uint32_t n_going = (uint32_t)__popcll(__ballot(going));
+        const uint64_t m_going = __ballot(going);
+        const uint32_t n_going = (uint32_t)__popcll(m_going);
         for (;;) {
             if (TL) {
                 // Instance transitions: leave a finished instance (not while
@@ -511,91 +531,98 @@ __device__ __forceinline__ void trav_run(const DevScene& sc, Trav& s, bool& goin
                 else s.leaf2 = s.node;
                 s.node = s.sp == 0 ? kNoChild : st.pop(s.sp);
             }
-            const bool inner = going && !(s.node & kLeafFlag) && s.node != kNoChild && (!TL || s.node != kInstMarker);
-            const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
+            // no leaf flag, not kNoChild (nor the instance marker): one unsigned
+            // compare, balloted as it is and masked with the going lanes
+            const bool below = s.node < (TL ? kInstMarker : kNoChild);
+            const bool inner = going && below;
+            const uint32_t n_inner = (uint32_t)__popcll(__ballot(below) & m_going);
             if (n_inner == 0) break;
             if (n_inner * 64u < (uint32_t)inner_frac * n_going) break;
-            if (!inner) continue;
-            if (COUNT) {
-                (*nn)++;
-                if (first_active_lane()) wi[0]++;
-            }
-            // one 4-wide node: six float4 of child bounds (SoA: lo.x, hi.x,
-            // lo.y, hi.y, lo.z, hi.z) + one of refs.  The ray loads each
-            // axis's entry and exit planes by its direction signs, so the
-            // slab test needs no min/max per axis.
-            // (32-bit byte offsets: the builder keeps the node array below 4 GiB)
-            typedef const float4 cf4;
-            const char* nb = (const char*)sc.nodes;
-            const uint32_t no = s.node * (16u * kWideNodeF4);
-            const TRay& r = s.r;
-            float4 ex, fx, ey, fy, ez, fz, rf;
-            if (TOPN > 0 && s.node < (uint32_t)TOPN) {
-                typedef const char __attribute__((address_space(3))) lchar;
-                lchar* lb = (lchar*)top;
-                ex = lld4(lb + (no + r.nx));
-                fx = lld4(lb + (no + (r.nx ^ 16u)));
-                ey = lld4(lb + (no + r.ny));
-                fy = lld4(lb + (no + (r.ny ^ 16u)));
-                ez = lld4(lb + (no + r.nz));
-                fz = lld4(lb + (no + (r.nz ^ 16u)));
-                rf = lld4(lb + (no + 96u));
-            } else {
-                ex = ld4<G>((cf4*)(nb + (no + r.nx)));
-                fx = ld4<G>((cf4*)(nb + (no + (r.nx ^ 16u))));
-                ey = ld4<G>((cf4*)(nb + (no + r.ny)));
-                fy = ld4<G>((cf4*)(nb + (no + (r.ny ^ 16u))));
-                ez = ld4<G>((cf4*)(nb + (no + r.nz)));
-                fz = ld4<G>((cf4*)(nb + (no + (r.nz ^ 16u))));
-                rf = ld4<G>((cf4*)(nb + (no + 96u)));
-            }
-            float key[4];
-            uint32_t ref[4];
-            int nhit = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float x0, x1, y0, y1, z0, z1;
-                if (FMA) {
-                    x0 = __builtin_fmaf(comp4(ex, j), r.idir.x, -r.ce.x);
-                    x1 = __builtin_fmaf(comp4(fx, j), r.idir.x, -r.cx.x);
-                    y0 = __builtin_fmaf(comp4(ey, j), r.idir.y, -r.ce.y);
-                    y1 = __builtin_fmaf(comp4(fy, j), r.idir.y, -r.cx.y);
-                    z0 = __builtin_fmaf(comp4(ez, j), r.idir.z, -r.ce.z);
-                    z1 = __builtin_fmaf(comp4(fz, j), r.idir.z, -r.cx.z);
-                } else {
-                    x0 = (comp4(ex, j) - r.o.x) * r.idir.x;
-                    x1 = (comp4(fx, j) - r.o.x) * r.idir.x;
-                    y0 = (comp4(ey, j) - r.o.y) * r.idir.y;
-                    y1 = (comp4(fy, j) - r.o.y) * r.idir.y;
-                    z0 = (comp4(ez, j) - r.o.z) * r.idir.z;
-                    z1 = (comp4(fz, j) - r.o.z) * r.idir.z;
+            // the node step as a block, not behind a `continue`: every exit of
+            // this loop is then wave-uniform (the two breaks above), and the
+            // loop's control stays in scalar registers (2-3 % faster frames)
+            if (inner) {
+                if (COUNT) {
+                    (*nn)++;
+                    if (first_active_lane()) wi[0]++;
                 }
-                const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
-                const float tf = fminf(fminf(x1, y1), fminf(z1, s.h.t));
-                // empty slots hold a point box at 1e38 that no slab test accepts
-                const uint32_t c = __float_as_uint(comp4(rf, j));
-                const bool hit = tn <= tf * kBoxPad;
-                key[j] = hit ? tn : __uint_as_float(0x7F800000u);
-                ref[j] = c;
-                nhit += hit ? 1 : 0;
-            }
-            // ascending by entry distance (misses last): 5 compare-exchanges
-            sort_ce(key[0], ref[0], key[1], ref[1]);
-            sort_ce(key[2], ref[2], key[3], ref[3]);
-            sort_ce(key[0], ref[0], key[2], ref[2]);
-            sort_ce(key[1], ref[1], key[3], ref[3]);
-            sort_ce(key[1], ref[1], key[2], ref[2]);
-            // visit the nearest; push the others farthest first (slot
-            // sp + nhit-1-j for child j), so the next nearest is on top.
-            // With no hit the top of the stack is next (read before any
-            // write: the writes go to slots >= sp).
-            const uint32_t top = st.step4(s.sp, nhit, ref[1], ref[2], ref[3]);
-            if (nhit == 0) {
-                s.node = s.sp == 0 ? kNoChild : top;
-                s.sp = s.sp > 0 ? s.sp - 1 : 0;
-            } else {
-                s.sp += nhit - 1;
-                s.node = ref[0];
+                // one 4-wide node: six float4 of child bounds (SoA: lo.x, hi.x,
+                // lo.y, hi.y, lo.z, hi.z) + one of refs.  The ray loads each
+                // axis's entry and exit planes by its direction signs, so the
+                // slab test needs no min/max per axis.
+                // (32-bit byte offsets: the builder keeps the node array below 4 GiB)
+                typedef const float4 cf4;
+                const char* nb = (const char*)sc.nodes;
+                const uint32_t no = s.node * (16u * kWideNodeF4);
+                const TRay& r = s.r;
+                float4 ex, fx, ey, fy, ez, fz, rf;
+                if (TOPN > 0 && s.node < (uint32_t)TOPN) {
+                    typedef const char __attribute__((address_space(3))) lchar;
+                    lchar* lb = (lchar*)top;
+                    ex = lld4(lb + (no + r.nx));
+                    fx = lld4(lb + (no + (r.nx ^ 16u)));
+                    ey = lld4(lb + (no + r.ny));
+                    fy = lld4(lb + (no + (r.ny ^ 16u)));
+                    ez = lld4(lb + (no + r.nz));
+                    fz = lld4(lb + (no + (r.nz ^ 16u)));
+                    rf = lld4(lb + (no + 96u));
+                } else {
+                    ex = ld4<G>((cf4*)(nb + (no + r.nx)));
+                    fx = ld4<G>((cf4*)(nb + (no + (r.nx ^ 16u))));
+                    ey = ld4<G>((cf4*)(nb + (no + r.ny)));
+                    fy = ld4<G>((cf4*)(nb + (no + (r.ny ^ 16u))));
+                    ez = ld4<G>((cf4*)(nb + (no + r.nz)));
+                    fz = ld4<G>((cf4*)(nb + (no + (r.nz ^ 16u))));
+                    rf = ld4<G>((cf4*)(nb + (no + 96u)));
+                }
+                float key[4];
+                uint32_t ref[4];
+                int nhit = 0;
+    #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float x0, x1, y0, y1, z0, z1;
+                    if (FMA) {
+                        x0 = __builtin_fmaf(comp4(ex, j), r.idir.x, -r.ce.x);
+                        x1 = __builtin_fmaf(comp4(fx, j), r.idir.x, -r.cx.x);
+                        y0 = __builtin_fmaf(comp4(ey, j), r.idir.y, -r.ce.y);
+                        y1 = __builtin_fmaf(comp4(fy, j), r.idir.y, -r.cx.y);
+                        z0 = __builtin_fmaf(comp4(ez, j), r.idir.z, -r.ce.z);
+                        z1 = __builtin_fmaf(comp4(fz, j), r.idir.z, -r.cx.z);
+                    } else {
+                        x0 = (comp4(ex, j) - r.o.x) * r.idir.x;
+                        x1 = (comp4(fx, j) - r.o.x) * r.idir.x;
+                        y0 = (comp4(ey, j) - r.o.y) * r.idir.y;
+                        y1 = (comp4(fy, j) - r.o.y) * r.idir.y;
+                        z0 = (comp4(ez, j) - r.o.z) * r.idir.z;
+                        z1 = (comp4(fz, j) - r.o.z) * r.idir.z;
+                    }
+                    const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
+                    const float tf = fminf(fminf(x1, y1), fminf(z1, s.h.t));
+                    // empty slots hold a point box at 1e38 that no slab test accepts
+                    const uint32_t c = __float_as_uint(comp4(rf, j));
+                    const bool hit = tn <= tf * kBoxPad;
+                    key[j] = hit ? tn : __uint_as_float(0x7F800000u);
+                    ref[j] = c;
+                    nhit += hit ? 1 : 0;
+                }
+                // ascending by entry distance (misses last): 5 compare-exchanges
+                sort_ce(key[0], ref[0], key[1], ref[1]);
+                sort_ce(key[2], ref[2], key[3], ref[3]);
+                sort_ce(key[0], ref[0], key[2], ref[2]);
+                sort_ce(key[1], ref[1], key[3], ref[3]);
+                sort_ce(key[1], ref[1], key[2], ref[2]);
+                // visit the nearest; push the others farthest first (slot
+                // sp + nhit-1-j for child j), so the next nearest is on top.
+                // With no hit the top of the stack is next (read before any
+                // write: the writes go to slots >= sp).
+                const uint32_t top = st.step4(s.sp, nhit, ref[1], ref[2], ref[3]);
+                if (nhit == 0) {
+                    s.node = s.sp == 0 ? kNoChild : top;
+                    s.sp = s.sp > 0 ? s.sp - 1 : 0;
+                } else {
+                    s.sp += nhit - 1;
+                    s.node = ref[0];
+                }
             }
         }
         // leaf phase (compacted when the whole wave is here)
