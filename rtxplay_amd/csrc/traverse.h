@@ -197,12 +197,18 @@ struct LaneStack {
     __device__ __forceinline__ uint32_t step4(int sp, int nhit, uint32_t r1, uint32_t r2, uint32_t r3) {
         if (sp + 3 <= SLDS) {
             const uint32_t top = lds[(sp > 0 ? sp - 1 : 0) * BLOCK];
-            const int s1 = nhit > 1 ? sp + nhit - 2 : sp + 2;
-            const int s2 = nhit > 2 ? sp + nhit - 3 : sp + 2;
-            const int s3 = nhit > 3 ? sp : sp + 2;
-            lds[s3 * BLOCK] = r3;
-            lds[s2 * BLOCK] = r2;
-            lds[s1 * BLOCK] = r1;
+            // slot addresses from one base: r1's slot, then one and two
+            // slots below it; a child not pushed goes to the free slot sp+2
+            // (a base below the column is never selected: p1 is chosen only
+            // for hit children, whose slots are >= sp)
+            lds_u32* const p1 = lds + (sp + nhit - 2) * BLOCK;
+            lds_u32* const fr = lds + (sp + 2) * BLOCK;
+            lds_u32* const a3 = nhit > 3 ? p1 - 2 * BLOCK : fr;
+            lds_u32* const a2 = nhit > 2 ? p1 - BLOCK : fr;
+            lds_u32* const a1 = nhit > 1 ? p1 : fr;
+            *a3 = r3;
+            *a2 = r2;
+            *a1 = r1;
             return top;
         }
         const uint32_t top = peek(sp);
@@ -365,17 +371,25 @@ __device__ __forceinline__ uint32_t max_dpp(uint32_t v) {
     return v > t ? v : t;
 }
 
+// v + v of the lane that the DPP control CTRL names (0 where max_dpp keeps v)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t add_dpp(uint32_t v) {
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
 template <bool G, bool TL, bool COUNT>
 __device__ __forceinline__ void leaves_compact(const DevScene& sc, Trav& s, uint32_t f1, uint32_t c1, uint32_t f2,
                                                uint32_t c2, lds_u32* cl, uint32_t* wi) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = c1 + c2;
+    // inclusive prefix sum of c over the wave (DPP, as the owner scan below)
     uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)incl, (unsigned)d, 64);
-        if (lane >= (uint32_t)d) incl += y;
-    }
+    incl = add_dpp<0x111, 0xf>(incl);   // row_shr:1
+    incl = add_dpp<0x112, 0xf>(incl);   // row_shr:2
+    incl = add_dpp<0x114, 0xf>(incl);   // row_shr:4
+    incl = add_dpp<0x118, 0xf>(incl);   // row_shr:8
+    incl = add_dpp<0x142, 0xa>(incl);   // row_bcast:15 into rows 1 and 3
+    incl = add_dpp<0x143, 0xc>(incl);   // row_bcast:31 into rows 2 and 3
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     if (total == 0) return;
     const uint32_t excl = incl - c;
