@@ -38,6 +38,11 @@ constexpr uint32_t kWideMinChunk = 2048;  // ... of this many primitives at leas
 constexpr uint32_t kRootBlocks = 2048;                   // blocks of the root bounds reduction at most
 constexpr int kScanBlock = 1024;
 // SAH constants (cost of one node visit vs one triangle test)
+// 1: the 4-wide tree is the SAH-optimal grouping of the binary one; 0: each
+// wide node opens its largest-area inner child until it has 4 children
+#ifndef RTX_DP_COLLAPSE
+#define RTX_DP_COLLAPSE 1
+#endif
 #ifndef RTX_SAH_TRAV
 #define RTX_SAH_TRAV 1.0f
 #endif
@@ -1038,9 +1043,119 @@ __global__ void k_collapse4(const BNode* __restrict__ bn, const uint32_t* __rest
     ninner[i] = m;
 }
 
+// ---------------------------------------------------------------------------
+// SAH-optimal collapse (after Ylitie, Karras and Laine 2017, "Efficient
+// incoherent ray traversal on GPUs through compressed wide BVHs", for 4-wide
+// nodes).  D(n, k) is the least SAH cost of the binary subtree at n as a
+// forest of at most k roots, each root a leaf (all the subtree's primitives,
+// at most max_leaf, contiguous in the leaf order) or a 4-wide node whose
+// children are a forest of at most 4 roots under it:
+//   D(n, 1) = min(A(n) * n_prims * C_isect  [n_prims <= max_leaf],
+//                 A(n) * C_trav + min_i D(L, i) + D(R, 4 - i))
+//   D(n, k) = min(D(n, 1), min_{0<i<k} D(L, i) + D(R, k - i))
+// bottom-up, one binary level at a time (the build allocates the nodes level
+// by level, so a level's children are all in the next one).  A binary leaf
+// stays a leaf; the root stays a 4-wide node.  The collapse then takes each
+// wide node's children from the choices (split of the 4 roots between L and
+// R, then each subtree's own split).  The binary tree and its statistics are
+// unchanged; only the grouping into 4-wide nodes and leaves is.
+//
+// meta bits: 0 the subtree is one leaf; 1-2 the split of 4 roots between L
+// and R (1..3 to L); 3-4, 5-6, 7-8 the split for k = 2, 3, 4 (0: the subtree
+// itself is the one root, else the roots given to L)
+__global__ void k_dp_level(const BNode* __restrict__ bn, uint32_t begin, uint32_t n, float4* __restrict__ dc,
+                           uint32_t* __restrict__ ntri, uint32_t* __restrict__ first, uint32_t* __restrict__ meta,
+                           uint32_t max_leaf) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = begin + t;
+    const BNode b = bn[i];
+    const float A = area3(b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z);
+    const uint32_t cnt = __float_as_uint(b.hi.w);
+    if (cnt) {
+        const float c = A * (float)cnt * kCostIsect;
+        dc[i] = make_float4(c, c, c, c);
+        ntri[i] = cnt;
+        first[i] = __float_as_uint(b.lo.w);
+        meta[i] = 1u;
+        return;
+    }
+    const uint32_t L = __float_as_uint(b.lo.w), R = L + 1u;
+    const float4 dl = dc[L], dr = dc[R];
+    const float l[5] = {0.f, dl.x, dl.y, dl.z, dl.w}, r[5] = {0.f, dr.x, dr.y, dr.z, dr.w};
+    float d2[5];
+    uint32_t b2[5];
+    for (int k = 2; k <= 4; k++) {
+        float best = l[1] + r[k - 1];
+        uint32_t bi = 1;
+        for (int s = 2; s < k; s++) {
+            const float c = l[s] + r[k - s];
+            if (c < best) { best = c; bi = (uint32_t)s; }
+        }
+        d2[k] = best;
+        b2[k] = bi;
+    }
+    const uint32_t nt = ntri[L] + ntri[R];
+    const float cnode = A * kCostTrav + d2[4];
+    const bool leaf = i != 0u && nt <= max_leaf && A * (float)nt * kCostIsect <= cnode;
+    const float d1 = leaf ? A * (float)nt * kCostIsect : cnode;
+    uint32_t m = (leaf ? 1u : 0u) | (b2[4] << 1);
+    float d[5];
+    d[1] = d1;
+    for (int k = 2; k <= 4; k++) {
+        const bool self = d1 <= d2[k];
+        d[k] = self ? d1 : d2[k];
+        m |= (self ? 0u : b2[k]) << (3 + 2 * (k - 2));
+    }
+    dc[i] = make_float4(d[1], d[2], d[3], d[4]);
+    ntri[i] = nt;
+    first[i] = first[L];
+    meta[i] = m;
+}
+
+// the children of the wide node at binary node front[i]: the forest of 4
+// roots that the choices give under it (left subtree first)
+__global__ void k_collapse4_dp(const BNode* __restrict__ bn, const uint32_t* __restrict__ front, uint32_t n,
+                               const uint32_t* __restrict__ meta, uint4* __restrict__ cid,
+                               uint32_t* __restrict__ ninner) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = front[i];
+    const uint32_t L = __float_as_uint(bn[b].lo.w);
+    const uint32_t kl = (meta[b] >> 1) & 3u;
+    // (subtree, roots) pairs still to expand; their roots add up to 4, so
+    // neither the stack nor the output holds more than 4
+    uint32_t sn[4], sk[4], out[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
+    int sp = 0, no = 0;
+    sn[sp] = L + 1u; sk[sp++] = 4u - kl;
+    sn[sp] = L; sk[sp++] = kl;
+    while (sp > 0) {
+        --sp;
+        const uint32_t x = sn[sp], k = sk[sp];
+        const uint32_t mx = meta[x];
+        const uint32_t s = k == 1u ? 0u : (mx >> (3u + 2u * (k - 2u))) & 3u;
+        if (s == 0u) {
+            out[no++] = x;
+        } else {
+            const uint32_t XL = __float_as_uint(bn[x].lo.w);
+            sn[sp] = XL + 1u; sk[sp++] = k - s;
+            sn[sp] = XL; sk[sp++] = s;
+        }
+    }
+    uint32_t m = 0;
+    for (int k = 0; k < no; k++) m += (meta[out[k]] & 1u) ? 0u : 1u;
+    cid[i] = make_uint4(out[0], out[1], out[2], out[3]);
+    ninner[i] = m;
+}
+
+// Writes the wide nodes of one level.  With meta (the SAH-optimal collapse)
+// a child is a leaf when its subtree is one (primitives first[c] ..
+// first[c] + ntri[c] - 1 of the leaf order); without, when it is a binary leaf.
 __global__ void k_write4(const BNode* __restrict__ bn, const uint4* __restrict__ cid, const uint32_t* __restrict__ off,
                          uint32_t n, uint32_t base_this, uint32_t base_next, uint32_t empty_ref,
-                         float4* __restrict__ nodes4, uint32_t* __restrict__ next_front) {
+                         float4* __restrict__ nodes4, uint32_t* __restrict__ next_front,
+                         const uint32_t* __restrict__ meta, const uint32_t* __restrict__ ntri,
+                         const uint32_t* __restrict__ first) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint4 c4 = cid[i];
@@ -1061,7 +1176,9 @@ __global__ void k_write4(const BNode* __restrict__ bn, const uint4* __restrict__
         const BNode b = bn[c];
         lx[k] = b.lo.x; hx[k] = b.hi.x; ly[k] = b.lo.y; hy[k] = b.hi.y; lz[k] = b.lo.z; hz[k] = b.hi.z;
         const uint32_t cntp = __float_as_uint(b.hi.w);
-        if (cntp) {
+        if (meta && (meta[c] & 1u)) {
+            ref[k] = leaf_ref(first[c], ntri[c]);
+        } else if (!meta && cntp) {
             ref[k] = leaf_ref(__float_as_uint(b.lo.w), cntp);
         } else {
             ref[k] = base_next + j;
@@ -1219,7 +1336,9 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
     uint32_t ntasks = 1, nnodes = 1;
     BuildTask* cur = tasks_a.p;
     BuildTask* nxt = tasks_b.p;
+    std::vector<uint32_t> level_first;   // binary level d: nodes level_first[d] .. level_first[d+1]-1
     while (ntasks > 0) {
+        level_first.push_back(nnodes - ntasks);
         // a few big tasks: each spread over G chunks (kWideBlocks in all, of
         // kWideMinChunk primitives at least on average), their bins merged by
         // k_merge_wide; a wave per task once tasks average fewer than
@@ -1286,6 +1405,29 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         RTX_TRY(hipGetLastError());
     }
 
+    level_first.push_back(nnodes);
+#if RTX_DP_COLLAPSE
+    // the collapse's choices, bottom-up by binary level
+    DevBuf<float4> dpc;
+    DevBuf<uint32_t> dp_ntri, dp_first, dp_meta;
+    if (ninner) {
+        RTX_TRY(dpc.alloc(nnodes));
+        RTX_TRY(dp_ntri.alloc(nnodes));
+        RTX_TRY(dp_first.alloc(nnodes));
+        RTX_TRY(dp_meta.alloc(nnodes));
+        for (size_t d = level_first.size() - 1; d-- > 0;) {
+            const uint32_t b = level_first[d], n = level_first[d + 1] - b;
+            if (n == 0) continue;
+            hipLaunchKernelGGL(k_dp_level, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, b, n, dpc.p,
+                               dp_ntri.p, dp_first.p, dp_meta.p, max_leaf);
+        }
+        RTX_TRY(hipGetLastError());
+    }
+    const uint32_t *wmeta = dp_meta.p, *wntri = dp_ntri.p, *wfirst = dp_first.p;
+#else
+    const uint32_t *wmeta = nullptr, *wntri = nullptr, *wfirst = nullptr;
+#endif
+
     // 4-wide nodes: at most one per binary inner node
     uint32_t nwide = 0, wide_depth = 0;
     if (ninner) {
@@ -1305,11 +1447,16 @@ hipError_t build_bvh(const float4* d_tri_world, const float4* d_plo, const float
         uint32_t* nxt = fb.p;
         while (n > 0) {
             out.level_begin.push_back(nwide);
+#if RTX_DP_COLLAPSE
+            hipLaunchKernelGGL(k_collapse4_dp, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cur, n, dp_meta.p,
+                               cid.p, cnt.p);
+#else
             hipLaunchKernelGGL(k_collapse4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cur, n, cid.p,
                                cnt.p);
+#endif
             scan_u32(cnt.p, n, off.p);
             hipLaunchKernelGGL(k_write4, dim3(blocks_for(n, 256)), dim3(256), 0, s, bnodes.p, cid.p, off.p, n,
-                               nwide, nwide + n, leaf_ref(N, 1), wide.p, nxt);
+                               nwide, nwide + n, leaf_ref(N, 1), wide.p, nxt, wmeta, wntri, wfirst);
             RTX_TRY(hipGetLastError());
             uint32_t nn = 0;
             RTX_TRY(hipMemcpyAsync(&nn, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
