@@ -895,7 +895,8 @@ def main():
                          "a plain streaming kernel of known size that calibrates the HBM byte counters")
     ap.add_argument("--counters", action="store_true",
                     help="run the rocprofv3 counter passes of the headline frame and write their summary")
-    ap.add_argument("--counters-out", default=os.path.join(ROOT, "gpurun_out", "counters"))
+    ap.add_argument("--counters-out", default=os.path.join(ROOT, "counters_out"),
+                    help="directory for the raw counter passes and the summary (default: counters_out/)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (pixels would be sharded "
