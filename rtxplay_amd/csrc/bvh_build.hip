@@ -1004,10 +1004,12 @@ __global__ void k_inner_flag(const BNode* __restrict__ bnodes, uint32_t nnodes, 
 }
 
 // ---------------------------------------------------------------------------
-// collapse of the binary tree into 4-wide nodes, level by level from the root.
-// A wide node starts from a binary node's two children and repeatedly opens
-// the inner child of largest surface area until it has 4 children or only
-// leaves; its inner children become the wide nodes of the next level.
+// collapse of the binary tree into 4-wide nodes, level by level from the root
+// (the grouping of builds with RTX_DP_COLLAPSE=0; the default is the
+// SAH-optimal one below).  A wide node starts from a binary node's two
+// children and repeatedly opens the inner child of largest surface area until
+// it has 4 children or only leaves; its inner children become the wide nodes
+// of the next level.
 __global__ void k_collapse4(const BNode* __restrict__ bn, const uint32_t* __restrict__ front, uint32_t n,
                             uint4* __restrict__ cid, uint32_t* __restrict__ ninner) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
