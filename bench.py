@@ -829,7 +829,11 @@ def frame_leg(args, level, w, h, spp, depth):
             "samples_per_s": round(samples, 1), "Msamples_per_s": round(samples / 1e6, 2), "rays": rays,
             "rays_per_sample": round(rays / (w * h * spp), 4),
             "bvh": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "max_depth": st["max_depth"],
-                    "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1)},
+                    "sah_cost": round(st["sah_cost"], 2), "build_ms": round(st["build_ms"], 1),
+                    "wide_nodes": st["wide_nodes"], "wide_depth": st["wide_depth"],
+                    "device_bytes": st["device_bytes"],
+                    "note": "nodes, leaves, depth and SAH cost of the binary SAH tree; wide_*: the 4-wide "
+                            "tree the kernels traverse (its SAH-optimal grouping)"},
             "roofline": rf}, raw
 
 
